@@ -1,0 +1,139 @@
+"""Generate the pass-kernel instantiation units and their registries.
+
+Each pass shape (a run of radix stages, stage 0 = unit sub-stride) is
+instantiated for the four pass kinds (A^T, A, conj(A), fused convolution)
+and both tile layouts (strided lines / contiguous lines), in FP64 and FP32.
+The registry rows let the C host pick a kernel by (precision, kind, layout,
+stage list).  Run by the package build; output goes to csrc/gen/.
+"""
+from __future__ import annotations
+
+import os
+import sys
+
+KINDS = ["PACG_K_DIF", "PACG_K_DIT", "PACG_K_DITC", "PACG_K_FUSED"]
+LAYOUTS = ["PACG_L_STRIDED", "PACG_L_CONTIG"]
+
+# pure-radix runs (every pass of a pure radix-r schedule) ...
+SHAPES = [[2] * k for k in range(1, 9)] + [[4] * k for k in range(1, 5)] + [[8] * k for k in range(1, 4)]
+# ... and the mixed runs of the mixed-radix schedules the planner uses:
+# 256 = 4*8*8 and 8*8*4, 2048 = 4^5 * 2 split as (4^4)(4, 2), 32 = 4*8 / 8*4
+SHAPES += [[4, 8, 8], [8, 8, 4], [4, 2], [2, 4], [4, 8], [8, 4], [8, 2], [2, 8]]
+
+E_TARGET = 16
+
+
+def phases(shape):
+    """Greedy bottom-up grouping of stages into register phases (product <= 16)."""
+    out = []
+    cur = 1
+    for g, r in enumerate(shape):
+        if cur > 1 and cur * r > E_TARGET:
+            out.append(g - 1)
+            cur = 1
+        cur *= r
+    out.append(len(shape) - 1)
+    return out
+
+
+def codes(shape):
+    lg = {2: 1, 4: 2, 8: 3}
+    sc = 0
+    for g, r in enumerate(shape):
+        sc |= lg[r] << (2 * g)
+    ph = phases(shape)
+    pc = 0
+    for p, top in enumerate(ph):
+        pc |= top << (4 * p)
+    return sc, len(ph), pc
+
+
+def geometry(shape, prec):
+    P = 1
+    for r in shape:
+        P *= r
+    ph = phases(shape)
+    E = 1
+    lo = 0
+    for top in ph:
+        R = 1
+        for r in shape[lo:top + 1]:
+            R *= r
+        E = max(E, R)
+        lo = top + 1
+    tpl = P // E
+    tj = 32 if prec == "float" else 16
+    strided_lines = max(tj, 256 // tpl) if tpl < 8 else tj
+    while strided_lines * tpl > 1024:
+        strided_lines //= 2
+    elem = 4 if prec == "float" else 8
+    while P * (strided_lines + 1) * 2 * elem > 80 * 1024 and strided_lines > 4:
+        strided_lines //= 2
+    nl = max(1, 256 // tpl)
+    ldp = P + P // 32 + 1
+    while nl > 1 and nl * ldp * 2 * elem > 40 * 1024:
+        nl //= 2
+    return P, E, tpl, strided_lines, nl
+
+
+NCHUNK = 6
+
+
+def emit(prec: str, outdir: str):
+    """Write NCHUNK translation units for one precision; returns their table names."""
+    T = prec
+    tag = "f64" if prec == "double" else "f32"
+    pconst = "PACG_F64" if prec == "double" else "PACG_F32"
+    spec_types = ["double"] if prec == "double" else ["float", "double"]
+    rows = []
+    for shape in SHAPES:
+        sc, npph, pc = codes(shape)
+        P, E, tpl, sl, nl = geometry(shape, prec)
+        shname = "Shape<%d, 0x%Xu, %d, 0x%Xu>" % (len(shape), sc, npph, pc)
+        sname = "x".join(str(r) for r in shape)
+        for kind in KINDS:
+            for layout in LAYOUTS:
+                ln = sl if layout == "PACG_L_STRIDED" else nl
+                specs = spec_types if kind == "PACG_K_FUSED" else [T]
+                for ts in specs:
+                    fn = "launch_pass<%s, %s, %s, %s, %s, %d>" % (T, ts, shname, kind, layout, ln)
+                    sp = "PACG_F64" if ts == "double" else "PACG_F32"
+                    nm = "%s_%s_%s_%s_%s" % (tag, kind[7:].lower(), "s" if layout == "PACG_L_STRIDED" else "c", sname,
+                                             "h64" if (kind == "PACG_K_FUSED" and ts == "double") else "h")
+                    rows.append('  {%s, %s, %s, %s, %d, 0x%Xu, 0x%Xu, %d, %d, %d, &%s, "%s"},'
+                                % (pconst, sp, kind, layout, len(shape), sc, pc, P, ln, ln * tpl, fn, nm))
+    names = []
+    for k in range(NCHUNK):
+        part = rows[k::NCHUNK]
+        nm = "pacg_variants_%s_%d" % (tag, k)
+        names.append((nm, len(part)))
+        lines = [
+            "// generated by gen_variants.py -- pass kernel instantiations (%s, part %d)" % (tag, k),
+            '#include "../pacg_pass.hpp"',
+            "using namespace pacg;",
+            'extern "C" {',
+            "extern pacg_variant_t %s[];" % nm,
+            "pacg_variant_t %s[] = {" % nm,
+        ] + part + ["};", "}"]
+        with open(os.path.join(outdir, "%s.hip" % nm), "w") as f:
+            f.write("\n".join(lines) + "\n")
+    return names
+
+
+def main(outdir: str):
+    os.makedirs(outdir, exist_ok=True)
+    tables = emit("double", outdir) + emit("float", outdir)
+    idx = ["/* generated by gen_variants.py -- registry of all pass kernels */",
+           '#include "../pacg_internal.h"']
+    for nm, _ in tables:
+        idx.append("extern pacg_variant_t %s[];" % nm)
+    idx.append("const pacg_variant_t* const pacg_variant_tables[] = {%s};" % ", ".join(nm for nm, _ in tables))
+    idx.append("const int pacg_variant_counts[] = {%s};" % ", ".join(str(c) for _, c in tables))
+    idx.append("const int pacg_num_variant_tables = %d;" % len(tables))
+    with open(os.path.join(outdir, "pacg_variants_index.c"), "w") as f:
+        f.write("\n".join(idx) + "\n")
+    print("generated %d pass variants in %d units" % (sum(c for _, c in tables), len(tables)))
+
+
+if __name__ == "__main__":
+    main(sys.argv[1] if len(sys.argv) > 1 else os.path.join(os.path.dirname(os.path.abspath(__file__)), "gen"))

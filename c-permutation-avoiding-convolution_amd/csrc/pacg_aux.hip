@@ -1,0 +1,216 @@
+// Auxiliary device kernels and the HIP runtime glue used by the C host layer.
+//
+//   K1  twiddle generation (reference twiddle_factors.c:17-131): FP64 on the
+//       device, rounded once to the working precision.
+//   K7  digit-reversal gather / scatter (reference index_reversals.c:85-94,
+//       used by the ordered transforms fft.c:12-71): the index map is the
+//       closed form, evaluated per element; no rho table is read.
+//   pointwise product (reference convolution.c:20-26) and precision conversion.
+#include <hip/hip_runtime.h>
+
+#include <stdint.h>
+
+#include "pacg_internal.h"
+
+namespace {
+
+template <typename T>
+__global__ void twiddle_kernel(T* __restrict__ out, uint32_t N, uint32_t count, uint32_t step) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  // w_N^e = exp(-2 pi i e / N): sincospi of the exact dyadic 2e/N
+  const uint64_t e = ((uint64_t)i * step) % N;
+  double s, c;
+  sincospi(2.0 * (double)e / (double)N, &s, &c);
+  out[2 * i] = (T)c;
+  out[2 * i + 1] = (T)(-s);
+}
+
+__device__ __forceinline__ uint32_t rev_axis(uint32_t i, int nst, const uint8_t* lg, uint32_t n) {
+  // rho(i) = sum_q d_q * n / k_q, d_q the stage-q digit of i
+  uint32_t out = 0, sh = 0;
+  for (int q = 0; q < nst; ++q) {
+    const uint32_t b = lg[q];
+    const uint32_t d = (i >> sh) & ((1u << b) - 1u);
+    sh += b;
+    out += d * (n >> sh);
+  }
+  return out;
+}
+
+template <typename T>
+__global__ void permute_kernel(const T* __restrict__ ir, const T* __restrict__ ii, T* __restrict__ orr,
+                               T* __restrict__ oi, pacg_perm_desc_t d, uint64_t total, int scatter) {
+  const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= total) return;
+  uint64_t rem = p, acc = 0, mul = 1;
+  for (int a = d.nd - 1; a >= 0; --a) {
+    const uint32_t i = (uint32_t)(rem % d.dims[a]);
+    rem /= d.dims[a];
+    acc += (uint64_t)rev_axis(i, d.nst[a], d.radix_log2[a], d.dims[a]) * mul;
+    mul *= d.dims[a];
+  }
+  if (scatter) {
+    orr[acc] = ir[p];
+    oi[acc] = ii[p];
+  } else {
+    orr[p] = ir[acc];
+    oi[p] = ii[acc];
+  }
+}
+
+template <typename T>
+__global__ void gather_rho_kernel(const T* __restrict__ ir, const T* __restrict__ ii, T* __restrict__ orr,
+                                  T* __restrict__ oi, const int* __restrict__ rho, uint64_t n) {
+  const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const uint64_t src = (uint32_t)rho[p];
+  orr[p] = ir[src];
+  oi[p] = ii[src];
+}
+
+template <typename T, typename W>
+__global__ void pointwise_kernel(T* __restrict__ zr, T* __restrict__ zi, const W* __restrict__ wr,
+                                 const W* __restrict__ wi, uint64_t n, T scale) {
+  const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const T a = zr[p], b = zi[p], c = (T)wr[p], dd = (T)wi[p];
+  zr[p] = (a * c - b * dd) * scale;
+  zi[p] = (a * dd + b * c) * scale;
+}
+
+template <typename X, typename Y>
+__global__ void convert_kernel(const X* __restrict__ xr, const X* __restrict__ xi, Y* __restrict__ yr,
+                               Y* __restrict__ yi, uint64_t n, X scale) {
+  const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  yr[p] = (Y)(xr[p] * scale);
+  yi[p] = (Y)(xi[p] * scale);
+}
+
+inline uint32_t blocks_for(uint64_t n, uint32_t t) { return (uint32_t)((n + t - 1) / t); }
+
+}  // namespace
+
+extern "C" {
+
+int pacg_gen_twiddles(void* out, int prec, uint32_t N, uint32_t count, uint32_t step, void* stream) {
+  if (count == 0) return 0;
+  if (prec == PACG_F64)
+    hipLaunchKernelGGL(twiddle_kernel<double>, dim3(blocks_for(count, 256)), dim3(256), 0, (hipStream_t)stream,
+                       (double*)out, N, count, step);
+  else
+    hipLaunchKernelGGL(twiddle_kernel<float>, dim3(blocks_for(count, 256)), dim3(256), 0, (hipStream_t)stream,
+                       (float*)out, N, count, step);
+  return (int)hipGetLastError();
+}
+
+int pacg_permute(const void* in_re, const void* in_im, void* out_re, void* out_im, int prec,
+                 const pacg_perm_desc_t* d, int scatter, void* stream) {
+  uint64_t total = 1;
+  for (int a = 0; a < d->nd; ++a) total *= d->dims[a];
+  if (total == 0) return 0;
+  if (prec == PACG_F64)
+    hipLaunchKernelGGL(permute_kernel<double>, dim3(blocks_for(total, 256)), dim3(256), 0, (hipStream_t)stream,
+                       (const double*)in_re, (const double*)in_im, (double*)out_re, (double*)out_im, *d, total,
+                       scatter);
+  else
+    hipLaunchKernelGGL(permute_kernel<float>, dim3(blocks_for(total, 256)), dim3(256), 0, (hipStream_t)stream,
+                       (const float*)in_re, (const float*)in_im, (float*)out_re, (float*)out_im, *d, total, scatter);
+  return (int)hipGetLastError();
+}
+
+int pacg_gather_rho(const void* in_re, const void* in_im, void* out_re, void* out_im, const int* rho, uint64_t n,
+                    int prec, void* stream) {
+  if (n == 0) return 0;
+  dim3 g(blocks_for(n, 256)), b(256);
+  if (prec == PACG_F64)
+    hipLaunchKernelGGL(gather_rho_kernel<double>, g, b, 0, (hipStream_t)stream, (const double*)in_re,
+                       (const double*)in_im, (double*)out_re, (double*)out_im, rho, n);
+  else
+    hipLaunchKernelGGL(gather_rho_kernel<float>, g, b, 0, (hipStream_t)stream, (const float*)in_re,
+                       (const float*)in_im, (float*)out_re, (float*)out_im, rho, n);
+  return (int)hipGetLastError();
+}
+
+int pacg_pointwise(void* zre, void* zim, const void* wre, const void* wim, int prec, int wprec, uint64_t n,
+                   double scale, void* stream) {
+  if (n == 0) return 0;
+  dim3 g(blocks_for(n, 256)), b(256);
+  hipStream_t s = (hipStream_t)stream;
+  if (prec == PACG_F64)
+    hipLaunchKernelGGL((pointwise_kernel<double, double>), g, b, 0, s, (double*)zre, (double*)zim,
+                       (const double*)wre, (const double*)wim, n, scale);
+  else if (wprec == PACG_F64)
+    hipLaunchKernelGGL((pointwise_kernel<float, double>), g, b, 0, s, (float*)zre, (float*)zim, (const double*)wre,
+                       (const double*)wim, n, (float)scale);
+  else
+    hipLaunchKernelGGL((pointwise_kernel<float, float>), g, b, 0, s, (float*)zre, (float*)zim, (const float*)wre,
+                       (const float*)wim, n, (float)scale);
+  return (int)hipGetLastError();
+}
+
+int pacg_convert(const void* xre, const void* xim, int xprec, void* yre, void* yim, int yprec, uint64_t n,
+                 double scale, void* stream) {
+  if (n == 0) return 0;
+  dim3 g(blocks_for(n, 256)), b(256);
+  hipStream_t s = (hipStream_t)stream;
+  if (xprec == PACG_F64 && yprec == PACG_F64)
+    hipLaunchKernelGGL((convert_kernel<double, double>), g, b, 0, s, (const double*)xre, (const double*)xim,
+                       (double*)yre, (double*)yim, n, scale);
+  else if (xprec == PACG_F64)
+    hipLaunchKernelGGL((convert_kernel<double, float>), g, b, 0, s, (const double*)xre, (const double*)xim,
+                       (float*)yre, (float*)yim, n, scale);
+  else if (yprec == PACG_F64)
+    hipLaunchKernelGGL((convert_kernel<float, double>), g, b, 0, s, (const float*)xre, (const float*)xim,
+                       (double*)yre, (double*)yim, n, (float)scale);
+  else
+    hipLaunchKernelGGL((convert_kernel<float, float>), g, b, 0, s, (const float*)xre, (const float*)xim,
+                       (float*)yre, (float*)yim, n, (float)scale);
+  return (int)hipGetLastError();
+}
+
+// ---- runtime glue ----------------------------------------------------------
+int pacg_dev_malloc(void** p, size_t bytes) { return (int)hipMalloc(p, bytes ? bytes : 1); }
+int pacg_dev_free(void* p) { return p ? (int)hipFree(p) : 0; }
+int pacg_memcpy_h2d(void* dst, const void* src, size_t bytes, void* stream) {
+  return (int)hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, (hipStream_t)stream);
+}
+int pacg_memcpy_d2h(void* dst, const void* src, size_t bytes, void* stream) {
+  return (int)hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, (hipStream_t)stream);
+}
+int pacg_memcpy_d2d(void* dst, const void* src, size_t bytes, void* stream) {
+  return (int)hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, (hipStream_t)stream);
+}
+int pacg_stream_sync(void* stream) { return (int)hipStreamSynchronize((hipStream_t)stream); }
+int pacg_stream_create(void** stream) { return (int)hipStreamCreateWithFlags((hipStream_t*)stream, hipStreamNonBlocking); }
+int pacg_stream_destroy(void* stream) { return (int)hipStreamDestroy((hipStream_t)stream); }
+int pacg_device_sync(void) { return (int)hipDeviceSynchronize(); }
+int pacg_get_device(int* dev) { return (int)hipGetDevice(dev); }
+int pacg_device_count(int* n) { return (int)hipGetDeviceCount(n); }
+int pacg_last_hip_error(void) { return (int)hipGetLastError(); }
+const char* pacg_hip_error_string(int code) { return hipGetErrorString((hipError_t)code); }
+int pacg_event_create(void** ev) { return (int)hipEventCreate((hipEvent_t*)ev); }
+int pacg_event_destroy(void* ev) { return (int)hipEventDestroy((hipEvent_t)ev); }
+int pacg_event_record(void* ev, void* stream) { return (int)hipEventRecord((hipEvent_t)ev, (hipStream_t)stream); }
+int pacg_event_elapsed_ms(float* ms, void* start, void* stop) {
+  return (int)hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)stop);
+}
+int pacg_graph_begin(void* stream) {
+  return (int)hipStreamBeginCapture((hipStream_t)stream, hipStreamCaptureModeThreadLocal);
+}
+int pacg_graph_end(void* stream, void** exec) {
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture((hipStream_t)stream, &g);
+  if (e != hipSuccess) return (int)e;
+  hipGraphExec_t x = nullptr;
+  e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+  hipGraphDestroy(g);
+  if (e != hipSuccess) return (int)e;
+  *exec = (void*)x;
+  return 0;
+}
+int pacg_graph_launch(void* exec, void* stream) { return (int)hipGraphLaunch((hipGraphExec_t)exec, (hipStream_t)stream); }
+int pacg_graph_destroy(void* exec) { return (int)hipGraphExecDestroy((hipGraphExec_t)exec); }
+
+}  // extern "C"

@@ -1,0 +1,581 @@
+// Multi-stage butterfly pass kernels for gfx950.
+//
+// A transform along one axis is a product of radix stages (reference
+// butterflies.c): stage q with radix r_q, l_q = prod_{s<q} r_s, k_q = r_q l_q
+// maps legs m of the block b at positions b k + m l + j by
+//   A^T (DIF, butterflies.c:70-97,217-275,505-632):  y = F_r x,  y_m *= w_k^{m j}
+//   A   (DIT, butterflies.c:12-39,99-156,277-389):   x_m *= w_k^{m j}, y = F_r x
+//   conj(A)   (butterflies.c:41-68,158-215,391-503): conjugate of the above.
+//
+// A *pass* executes a run of consecutive stages [q_lo, q_hi] of one axis.
+// Its P = prod r_q points of one line sit S elements apart; the pass reads
+// every line once, does all its stages in registers and LDS, and writes the
+// line back to the same positions.  Two identities keep the twiddles cheap
+// (both exact in real arithmetic):
+//   * the stages of a pass equal a P-point DIF on the line followed by the
+//     diagonal w_K^{j' k(s)}, K = P l_{q_lo}, j' the within-axis position below
+//     the pass and k(s) the mixed digit reversal of the point index s;
+//   * the same holds one level down: a register *phase* (stages one thread
+//     holds together) is an R-point DIF with compile-time twiddles followed by
+//     w_{R lambda}^{mu k(r)} with mu the thread's lower sub-index.
+// DIT passes are the transposes: diagonal first, then the stages bottom up.
+// The fused pass of the convolution runs A^T, multiplies by the spectrum at
+// the same positions and runs conj(A) -- positions never move, so no
+// digit-reversal permutation is ever executed.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <stdint.h>
+#include <utility>
+
+#include "pacg_internal.h"
+
+namespace pacg {
+
+// ---------------------------------------------------------------------------
+// compile-time roots of unity: cos/sin(2 pi x / 64), octant-reduced Taylor sums
+// ---------------------------------------------------------------------------
+constexpr double kPi = 3.14159265358979323846;
+
+constexpr double sin_poly(double x) {
+  double x2 = x * x, term = x, sum = x;
+  for (int k = 1; k <= 13; ++k) {
+    term = -term * x2 / ((2.0 * k) * (2.0 * k + 1.0));
+    sum += term;
+  }
+  return sum;
+}
+constexpr double cos_poly(double x) {
+  double x2 = x * x, term = 1.0, sum = 1.0;
+  for (int k = 1; k <= 13; ++k) {
+    term = -term * x2 / ((2.0 * k - 1.0) * (2.0 * k));
+    sum += term;
+  }
+  return sum;
+}
+constexpr double cs64(int x, bool want_sin) {
+  x &= 63;
+  int q = x >> 4, rem = x & 15;
+  double c = 0.0, s = 0.0;
+  if (rem == 0) {
+    c = 1.0;
+    s = 0.0;
+  } else if (rem == 8) {
+    c = 0.70710678118654752440;
+    s = 0.70710678118654752440;
+  } else if (rem < 8) {
+    c = cos_poly(rem * kPi / 32.0);
+    s = sin_poly(rem * kPi / 32.0);
+  } else {
+    c = sin_poly((16 - rem) * kPi / 32.0);
+    s = cos_poly((16 - rem) * kPi / 32.0);
+  }
+  for (int i = 0; i < q; ++i) {
+    double t = c;
+    c = -s;
+    s = t;
+  }
+  return want_sin ? s : c;
+}
+
+// ---------------------------------------------------------------------------
+// compile-time loops
+// ---------------------------------------------------------------------------
+template <typename F, int... I>
+__device__ __forceinline__ void sfor_impl(F&& f, std::integer_sequence<int, I...>) {
+  (f(std::integral_constant<int, I>{}), ...);
+}
+template <int N, typename F>
+__device__ __forceinline__ void sfor(F&& f) {
+  sfor_impl(f, std::make_integer_sequence<int, N>{});
+}
+
+// ---------------------------------------------------------------------------
+// pass shape: NS stages (2 bits each: log2 radix, stage 0 = unit sub-stride),
+// NP register phases (4 bits each: index of the phase's top stage)
+// ---------------------------------------------------------------------------
+template <int NS_, uint32_t SC, int NP_, uint32_t PC>
+struct Shape {
+  static constexpr int NS = NS_;
+  static constexpr int NP = NP_;
+  static constexpr int lg(int g) { return (int)((SC >> (2 * g)) & 3u); }
+  static constexpr int rho(int g) { return 1 << lg(g); }
+  static constexpr int lglam(int g) {
+    int v = 0;
+    for (int h = 0; h < g; ++h) v += lg(h);
+    return v;
+  }
+  static constexpr int LGP = lglam(NS);
+  static constexpr int P = 1 << LGP;
+  static constexpr int phi(int p) { return (int)((PC >> (4 * p)) & 15u); }
+  static constexpr int plo(int p) { return p == 0 ? 0 : phi(p - 1) + 1; }
+  static constexpr int lgR(int p) { return lglam(phi(p) + 1) - lglam(plo(p)); }
+  static constexpr int lgE() {
+    int m = 0;
+    for (int p = 0; p < NP; ++p) m = lgR(p) > m ? lgR(p) : m;
+    return m;
+  }
+  static constexpr int E = 1 << lgE();
+  static constexpr int TPL = P / E;
+  // digit reversal of the local index r within phase p (mixed radix)
+  static constexpr int rev_phase(int p, int r) {
+    int lo = plo(p), out = 0;
+    int R = 1 << lgR(p);
+    for (int g = lo; g <= phi(p); ++g) {
+      int sh = lglam(g) - lglam(lo);
+      int d = (r >> sh) & (rho(g) - 1);
+      int kk = 1 << (lglam(g + 1) - lglam(lo));
+      out += d * (R / kk);
+    }
+    return out;
+  }
+  static constexpr bool valid() {
+    if (NS < 1 || NS > 12 || NP < 1 || phi(NP - 1) != NS - 1) return false;
+    for (int p = 0; p < NP; ++p)
+      if (phi(p) < plo(p)) return false;
+    for (int g = 0; g < NS; ++g)
+      if (lg(g) == 0) return false;
+    return E <= 16 && P <= 4096;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// complex helpers
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ void cmul(T& ar, T& ai, T br, T bi) {
+  T r = ar * br - ai * bi;
+  T i = ar * bi + ai * br;
+  ar = r;
+  ai = i;
+}
+
+// multiply v by exp(-2 pi i x / 64), conjugated when CJ; exact for quarter turns
+template <typename T, int X, bool CJ>
+__device__ __forceinline__ void rot64(T& vr, T& vi) {
+  constexpr int x = X & 63;
+  if constexpr (x == 0) {
+    return;
+  } else if constexpr (x == 16) {  // -i (or +i)
+    T r = vr;
+    if constexpr (!CJ) {
+      vr = vi;
+      vi = -r;
+    } else {
+      vr = -vi;
+      vi = r;
+    }
+  } else if constexpr (x == 32) {
+    vr = -vr;
+    vi = -vi;
+  } else if constexpr (x == 48) {  // +i (or -i)
+    T r = vr;
+    if constexpr (!CJ) {
+      vr = -vi;
+      vi = r;
+    } else {
+      vr = vi;
+      vi = -r;
+    }
+  } else {
+    constexpr double c = cs64(x, false);
+    constexpr double s = CJ ? cs64(x, true) : -cs64(x, true);
+    cmul<T>(vr, vi, (T)c, (T)s);
+  }
+}
+
+// In-place R-point DFT (R in 2,4,8) on v[B], v[B+ST], ..., natural order;
+// INV selects exp(+2 pi i / R) (conjugate butterflies).
+template <typename T, int R, bool INV, int B, int ST>
+__device__ __forceinline__ void dft(T* vr, T* vi) {
+  if constexpr (R == 2) {
+    T ar = vr[B], ai = vi[B], br = vr[B + ST], bi = vi[B + ST];
+    vr[B] = ar + br;
+    vi[B] = ai + bi;
+    vr[B + ST] = ar - br;
+    vi[B + ST] = ai - bi;
+  } else if constexpr (R == 4) {
+    T t0r = vr[B] + vr[B + 2 * ST], t0i = vi[B] + vi[B + 2 * ST];
+    T t1r = vr[B] - vr[B + 2 * ST], t1i = vi[B] - vi[B + 2 * ST];
+    T t2r = vr[B + ST] + vr[B + 3 * ST], t2i = vi[B + ST] + vi[B + 3 * ST];
+    T t3r = vr[B + ST] - vr[B + 3 * ST], t3i = vi[B + ST] - vi[B + 3 * ST];
+    vr[B] = t0r + t2r;
+    vi[B] = t0i + t2i;
+    vr[B + 2 * ST] = t0r - t2r;
+    vi[B + 2 * ST] = t0i - t2i;
+    // forward: y1 = t1 - i t3, y3 = t1 + i t3
+    if constexpr (!INV) {
+      vr[B + ST] = t1r + t3i;
+      vi[B + ST] = t1i - t3r;
+      vr[B + 3 * ST] = t1r - t3i;
+      vi[B + 3 * ST] = t1i + t3r;
+    } else {
+      vr[B + ST] = t1r - t3i;
+      vi[B + ST] = t1i + t3r;
+      vr[B + 3 * ST] = t1r + t3i;
+      vi[B + 3 * ST] = t1i - t3r;
+    }
+  } else {
+    static_assert(R == 8, "radix");
+    // even / odd halves: E = F4(x0,x2,x4,x6), O = F4(x1,x3,x5,x7)
+    dft<T, 4, INV, B, 2 * ST>(vr, vi);
+    dft<T, 4, INV, B + ST, 2 * ST>(vr, vi);
+    // y_p = E_p + w8^p O_p, y_{p+4} = E_p - w8^p O_p ; E_p at B+2p*ST, O_p at B+(2p+1)*ST
+    T er[4], ei[4], orr[4], oi[4];
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+      er[p] = vr[B + 2 * p * ST];
+      ei[p] = vi[B + 2 * p * ST];
+      orr[p] = vr[B + (2 * p + 1) * ST];
+      oi[p] = vi[B + (2 * p + 1) * ST];
+    }
+    rot64<T, 8, INV>(orr[1], oi[1]);
+    rot64<T, 16, INV>(orr[2], oi[2]);
+    rot64<T, 24, INV>(orr[3], oi[3]);
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+      vr[B + p * ST] = er[p] + orr[p];
+      vi[B + p * ST] = ei[p] + oi[p];
+      vr[B + (p + 4) * ST] = er[p] - orr[p];
+      vi[B + (p + 4) * ST] = ei[p] - oi[p];
+    }
+  }
+}
+
+enum { M_DIF = 0, M_DIT = 1, M_DITC = 2 };
+
+// One stage G of phase PH applied to every group the thread holds.
+template <typename T, class SH, int PH, int G, int MODE>
+__device__ __forceinline__ void stage(T* vr, T* vi) {
+  constexpr int lgR = SH::lgR(PH);
+  constexpr int R = 1 << lgR;
+  constexpr int lo = SH::plo(PH);
+  constexpr int lgl = SH::lglam(G) - SH::lglam(lo);
+  constexpr int rho = SH::rho(G);
+  constexpr int NGRP = SH::E / R;
+  constexpr int ST = 1 << lgl;
+  constexpr int M = rho << lgl;  // modulus of the internal twiddle
+  sfor<NGRP>([&](auto C_) {
+    sfor<R / rho>([&](auto B_) {
+      constexpr int bidx = decltype(B_)::value;
+      constexpr int mu = bidx & (ST - 1);
+      constexpr int hi = bidx >> lgl;
+      constexpr int base = decltype(C_)::value * R + hi * M + mu;
+      if constexpr (MODE != M_DIF) {
+        sfor<rho>([&](auto m_) {
+          constexpr int m = decltype(m_)::value;
+          rot64<T, (m * mu) * (64 / M), MODE == M_DITC>(vr[base + m * ST], vi[base + m * ST]);
+        });
+      }
+      dft<T, rho, MODE == M_DITC, base, ST>(vr, vi);
+      if constexpr (MODE == M_DIF) {
+        sfor<rho>([&](auto m_) {
+          constexpr int m = decltype(m_)::value;
+          rot64<T, (m * mu) * (64 / M), false>(vr[base + m * ST], vi[base + m * ST]);
+        });
+      }
+    });
+  });
+}
+
+// All stages of phase PH: DIF top-down, DIT bottom-up.
+template <typename T, class SH, int PH, int MODE>
+__device__ __forceinline__ void phase_stages(T* vr, T* vi) {
+  constexpr int lo = SH::plo(PH), hi = SH::phi(PH);
+  constexpr int cnt = hi - lo + 1;
+  sfor<cnt>([&](auto i_) {
+    constexpr int i = decltype(i_)::value;
+    constexpr int g = (MODE == M_DIF) ? (hi - i) : (lo + i);
+    stage<T, SH, PH, g, MODE>(vr, vi);
+  });
+}
+
+// Phase-external twiddle w_{R lambda}^{mu k(r)} from the w_P table in LDS.
+template <typename T, class SH, int PH, bool CJ>
+__device__ __forceinline__ void phase_twiddle(T* vr, T* vi, const T* twr, const T* twi, int tl) {
+  constexpr int lo = SH::plo(PH);
+  if constexpr (lo > 0) {
+    constexpr int lgR = SH::lgR(PH);
+    constexpr int R = 1 << lgR;
+    constexpr int lgl = SH::lglam(lo);
+    constexpr int sh = SH::LGP - SH::lglam(SH::phi(PH) + 1);
+    sfor<SH::E / R>([&](auto C_) {
+      constexpr int c = decltype(C_)::value;
+      const int o = tl + SH::TPL * c;
+      const int mu = o & ((1 << lgl) - 1);
+      sfor<R>([&](auto r_) {
+        constexpr int r = decltype(r_)::value;
+        constexpr int kr = SH::rev_phase(PH, r);
+        if constexpr (kr != 0) {
+          const int idx = (mu * kr) << sh;
+          T wr = twr[idx], wi = twi[idx];
+          if constexpr (CJ) wi = -wi;
+          cmul<T>(vr[c * R + r], vi[c * R + r], wr, wi);
+        }
+      });
+    });
+  }
+}
+
+// Position (within the line) of element (c, r) of thread tl in phase PH.
+template <class SH, int PH>
+__device__ __forceinline__ int pos(int tl, int c, int r) {
+  constexpr int lgl = SH::lglam(SH::plo(PH));
+  constexpr int lgR = SH::lgR(PH);
+  const int o = tl + SH::TPL * c;
+  const int mu = o & ((1 << lgl) - 1);
+  const int beta = o >> lgl;
+  return (beta << (lgl + lgR)) | (r << lgl) | mu;
+}
+
+// mixed digit reversal of a point index s over all stages of the pass
+template <class SH>
+__device__ __forceinline__ int rev_pass(int s) {
+  int out = 0;
+  sfor<SH::NS>([&](auto g_) {
+    constexpr int g = decltype(g_)::value;
+    const int d = (s >> SH::lglam(g)) & (SH::rho(g) - 1);
+    out |= d << (SH::LGP - SH::lglam(g + 1));
+  });
+  return out;
+}
+
+__device__ __forceinline__ uint32_t xcd_remap(uint32_t bid, uint32_t nb) {
+  // Workgroups are dealt round-robin to the 8 XCDs; give each XCD one
+  // contiguous range of tiles so neighbouring tiles share its L2.
+  if ((nb & 7u) != 0u) return bid;
+  return (bid & 7u) * (nb >> 3) + (bid >> 3);
+}
+
+template <typename T>
+struct Vec2;
+template <>
+struct Vec2<float> {
+  using type = float2;
+};
+template <>
+struct Vec2<double> {
+  using type = double2;
+};
+
+// ---------------------------------------------------------------------------
+// the pass kernel
+// ---------------------------------------------------------------------------
+template <typename T, typename TS, class SH, int KIND, int LAYOUT, int LINES>
+__global__ void __launch_bounds__(LINES* SH::TPL)
+    pass_kernel(const pacg_pass_args_t a) {
+  static_assert(SH::valid(), "invalid pass shape");
+  constexpr int P = SH::P, E = SH::E, TPL = SH::TPL, NP = SH::NP;
+  constexpr int NT = LINES * TPL;
+  // LDS: strided tiles are [P][LINES+1], contiguous tiles [LINES][P + P/32 + 1]
+  constexpr int LDJ = LINES + 1;
+  constexpr int LDP = P + P / 32 + 1;
+  constexpr int LDS_N = (LAYOUT == PACG_L_STRIDED) ? P * LDJ : LINES * LDP;
+  constexpr bool NEED_X = NP > 1;
+  __shared__ T xr[NEED_X ? LDS_N : 1];
+  __shared__ T xi[NEED_X ? LDS_N : 1];
+  __shared__ T twr[P], twi[P];
+
+  const int tid = threadIdx.x;
+  int tl, ll;
+  if constexpr (LAYOUT == PACG_L_STRIDED) {
+    ll = tid % LINES;
+    tl = tid / LINES;
+  } else {
+    ll = tid / TPL;
+    tl = tid % TPL;
+  }
+  const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
+  const uint32_t S = 1u << a.log2_S;
+  uint32_t line_base;  // element offset of point 0 of this thread's line
+  uint32_t jpos = 0;   // j of this line
+  bool valid;
+  if constexpr (LAYOUT == PACG_L_STRIDED) {
+    const uint32_t lg = tile * LINES + ll;
+    const uint32_t o = lg >> a.log2_S;
+    jpos = lg & (S - 1u);
+    valid = o < a.n_o;
+    line_base = o * (uint32_t)P * S + jpos;
+  } else {
+    const uint32_t line = tile * LINES + ll;
+    valid = line < a.n_o;
+    line_base = line * (uint32_t)P;
+  }
+  const uint32_t stride = (LAYOUT == PACG_L_STRIDED) ? S : 1u;
+
+  // w_P table -> LDS
+  {
+    using V2 = typename Vec2<T>::type;
+    const V2* tp = reinterpret_cast<const V2*>(a.tw_p);
+    for (int i = tid; i < P; i += NT) {
+      V2 w = tp[i];
+      twr[i] = w.x;
+      twi[i] = w.y;
+    }
+  }
+
+  const T* __restrict__ inr = reinterpret_cast<const T*>(a.in_re);
+  const T* __restrict__ ini = reinterpret_cast<const T*>(a.in_im);
+  T* __restrict__ outr = reinterpret_cast<T*>(a.out_re);
+  T* __restrict__ outi = reinterpret_cast<T*>(a.out_im);
+
+  T vr[E], vi[E];
+
+  auto lds_index = [&](int s) -> int {
+    if constexpr (LAYOUT == PACG_L_STRIDED)
+      return s * LDJ + ll;
+    else
+      return ll * LDP + s + (s >> 5);
+  };
+
+  auto load_phase = [&](auto PH_, const T* __restrict__ br, const T* __restrict__ bi) {
+    constexpr int PH = decltype(PH_)::value;
+    constexpr int R = 1 << SH::lgR(PH);
+    sfor<E / R>([&](auto c_) {
+      constexpr int c = decltype(c_)::value;
+      sfor<R>([&](auto r_) {
+        constexpr int r = decltype(r_)::value;
+        const int s = pos<SH, PH>(tl, c, r);
+        const uint32_t off = line_base + (uint32_t)s * stride;
+        vr[c * R + r] = valid ? br[off] : T(0);
+        vi[c * R + r] = valid ? bi[off] : T(0);
+      });
+    });
+  };
+  auto store_phase = [&](auto PH_, T scale) {
+    constexpr int PH = decltype(PH_)::value;
+    constexpr int R = 1 << SH::lgR(PH);
+    sfor<E / R>([&](auto c_) {
+      constexpr int c = decltype(c_)::value;
+      sfor<R>([&](auto r_) {
+        constexpr int r = decltype(r_)::value;
+        const int s = pos<SH, PH>(tl, c, r);
+        const uint32_t off = line_base + (uint32_t)s * stride;
+        if (valid) {
+          outr[off] = vr[c * R + r] * scale;
+          outi[off] = vi[c * R + r] * scale;
+        }
+      });
+    });
+  };
+  auto exchange = [&](auto FROM_, auto TO_) {
+    constexpr int FROM = decltype(FROM_)::value;
+    constexpr int TO = decltype(TO_)::value;
+    constexpr int RF = 1 << SH::lgR(FROM);
+    constexpr int RT = 1 << SH::lgR(TO);
+    __syncthreads();
+    sfor<E / RF>([&](auto c_) {
+      constexpr int c = decltype(c_)::value;
+      sfor<RF>([&](auto r_) {
+        constexpr int r = decltype(r_)::value;
+        const int li = lds_index(pos<SH, FROM>(tl, c, r));
+        xr[li] = vr[c * RF + r];
+        xi[li] = vi[c * RF + r];
+      });
+    });
+    __syncthreads();
+    sfor<E / RT>([&](auto c_) {
+      constexpr int c = decltype(c_)::value;
+      sfor<RT>([&](auto r_) {
+        constexpr int r = decltype(r_)::value;
+        const int li = lds_index(pos<SH, TO>(tl, c, r));
+        vr[c * RT + r] = xr[li];
+        vi[c * RT + r] = xi[li];
+      });
+    });
+  };
+  // pass-external twiddle w_K^{j' k(s)} on the phase-0 layout
+  auto ext_twiddle = [&](bool cj) {
+    using V2 = typename Vec2<T>::type;
+    const V2* hi = reinterpret_cast<const V2*>(a.tw_hi);
+    const V2* lo = reinterpret_cast<const V2*>(a.tw_lo);
+    const uint32_t jp = jpos >> a.log2_inner;
+    const uint32_t mask = (1u << a.lo_bits) - 1u;
+    constexpr int R0 = 1 << SH::lgR(0);
+    sfor<E / R0>([&](auto c_) {
+      constexpr int c = decltype(c_)::value;
+      sfor<R0>([&](auto r_) {
+        constexpr int r = decltype(r_)::value;
+        const int s = pos<SH, 0>(tl, c, r);
+        const uint32_t e = (jp * (uint32_t)rev_pass<SH>(s)) << a.ext_shift;
+        const V2 h = hi[e >> a.lo_bits];
+        const V2 l = lo[e & mask];
+        T wr = h.x * l.x - h.y * l.y;
+        T wi = h.x * l.y + h.y * l.x;
+        if (cj) wi = -wi;
+        cmul<T>(vr[c * R0 + r], vi[c * R0 + r], wr, wi);
+      });
+    });
+  };
+
+  if constexpr (KIND == PACG_K_DIF || KIND == PACG_K_FUSED) {
+    load_phase(std::integral_constant<int, NP - 1>{}, inr, ini);
+    __syncthreads();  // twiddle table visible
+    sfor<NP>([&](auto i_) {
+      constexpr int PH = NP - 1 - decltype(i_)::value;
+      if constexpr (PH < NP - 1) exchange(std::integral_constant<int, PH + 1>{}, std::integral_constant<int, PH>{});
+      phase_stages<T, SH, PH, M_DIF>(vr, vi);
+      phase_twiddle<T, SH, PH, false>(vr, vi, twr, twi, tl);
+    });
+    if constexpr (KIND == PACG_K_DIF) {
+      if (a.ext) ext_twiddle(false);
+      store_phase(std::integral_constant<int, 0>{}, (T)a.out_scale);
+    } else {
+      // spectral product at the same positions: x <- (x * h) * scale
+      const TS* __restrict__ hr = reinterpret_cast<const TS*>(a.spec_re);
+      const TS* __restrict__ hi = reinterpret_cast<const TS*>(a.spec_im);
+      constexpr int R0 = 1 << SH::lgR(0);
+      const T sc = (T)a.scale;
+      sfor<E / R0>([&](auto c_) {
+        constexpr int c = decltype(c_)::value;
+        sfor<R0>([&](auto r_) {
+          constexpr int r = decltype(r_)::value;
+          const int s = pos<SH, 0>(tl, c, r);
+          const uint32_t off = line_base + (uint32_t)s * stride;
+          T wr = valid ? (T)hr[off] : T(0);
+          T wi = valid ? (T)hi[off] : T(0);
+          cmul<T>(vr[c * R0 + r], vi[c * R0 + r], wr, wi);
+          vr[c * R0 + r] *= sc;
+          vi[c * R0 + r] *= sc;
+        });
+      });
+      sfor<NP>([&](auto i_) {
+        constexpr int PH = decltype(i_)::value;
+        if constexpr (PH > 0) exchange(std::integral_constant<int, PH - 1>{}, std::integral_constant<int, PH>{});
+        phase_twiddle<T, SH, PH, true>(vr, vi, twr, twi, tl);
+        phase_stages<T, SH, PH, M_DITC>(vr, vi);
+      });
+      store_phase(std::integral_constant<int, NP - 1>{}, (T)a.out_scale);
+    }
+  } else {
+    constexpr int MODE = (KIND == PACG_K_DITC) ? M_DITC : M_DIT;
+    load_phase(std::integral_constant<int, 0>{}, inr, ini);
+    __syncthreads();
+    if (a.ext) ext_twiddle(MODE == M_DITC);
+    sfor<NP>([&](auto i_) {
+      constexpr int PH = decltype(i_)::value;
+      if constexpr (PH > 0) exchange(std::integral_constant<int, PH - 1>{}, std::integral_constant<int, PH>{});
+      phase_twiddle<T, SH, PH, MODE == M_DITC>(vr, vi, twr, twi, tl);
+      phase_stages<T, SH, PH, MODE>(vr, vi);
+    });
+    store_phase(std::integral_constant<int, NP - 1>{}, (T)a.out_scale);
+  }
+}
+
+template <typename T, typename TS, class SH, int KIND, int LAYOUT, int LINES>
+int launch_pass(const pacg_pass_args_t* a, void* stream) {
+  uint32_t blocks;
+  if (LAYOUT == PACG_L_STRIDED) {
+    const uint64_t lines = (uint64_t)a->n_o << a->log2_S;
+    blocks = (uint32_t)((lines + LINES - 1) / LINES);
+  } else {
+    blocks = (a->n_o + LINES - 1) / LINES;
+  }
+  if (blocks == 0) return 0;
+  hipLaunchKernelGGL((pass_kernel<T, TS, SH, KIND, LAYOUT, LINES>), dim3(blocks), dim3(LINES * SH::TPL), 0,
+                     (hipStream_t)stream, *a);
+  return (int)hipGetLastError();
+}
+
+}  // namespace pacg

@@ -1,0 +1,127 @@
+/*
+ * pacg.h -- MI355X (gfx950) permutation-avoiding FFT convolution, extended C API.
+ *
+ * The drop-in entry points with the reference's own names live in
+ * pacg_fft.h; this header is the plan-based interface beneath them: device
+ * buffers, explicit mixed-radix schedules, FP64 or FP32 working precision,
+ * a cached kernel spectrum, error codes and a HIP stream.  No HIP or torch
+ * types appear in any signature; streams are passed as void*.
+ *
+ * Layout: split complex (separate re / im arrays), row-major, dims[nd-1]
+ * contiguous -- the reference's layout (butterflies_2d.c:23, butterflies_3d.c:25).
+ *
+ * Operators (reference notation, fft.h:17-231):
+ *   A        DIT butterflies            apply_butterflies_r*          (butterflies.c:12)
+ *   conj(A)  conjugate butterflies      apply_conjugate_butterflies_r* (butterflies.c:41)
+ *   A^T      transposed butterflies     apply_transposed_butterflies_r*(butterflies.c:70)
+ * with A P = F (DFT) for the schedule's digit reversal P, so A^T x = P F x is
+ * the DFT in digit-reversed order.  The convolution
+ *   y = (1/n) conj(A) [ (A^T k) .* (A^T x) ]          (convolution.c:31-72)
+ * never executes P.
+ */
+#ifndef PACG_H
+#define PACG_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes ---------------------------------------------------------- */
+enum {
+  PACG_OK = 0,
+  PACG_ERR_ARG = 1,       /* invalid argument (dims, radix, schedule, NULL) */
+  PACG_ERR_SIZE = 2,      /* total size >= 2^31 or an axis not equal to its schedule product */
+  PACG_ERR_NOMEM = 3,     /* host or device allocation failed */
+  PACG_ERR_HIP = 4,       /* a HIP runtime call failed (message names it) */
+  PACG_ERR_NOKERNEL = 5,  /* no compiled pass kernel covers the requested schedule */
+  PACG_ERR_STATE = 6      /* e.g. convolve before a kernel spectrum was set */
+};
+
+enum { PACG_PREC_F64 = 0, PACG_PREC_F32 = 1 };
+
+/* operations of pacg_execute (reference fft.c:12-88 and butterflies.c) */
+enum {
+  PACG_OP_TRANS = 0,  /* A^T            (transposed butterflies; fft2 without P) */
+  PACG_OP_CONJ = 1,   /* conj(A)        (conjugate butterflies)                  */
+  PACG_OP_FWD = 2,    /* A              (butterflies; ufft)                      */
+  PACG_OP_FFT = 3,    /* A P            (fft: natural-order DFT)                 */
+  PACG_OP_FFT2 = 4,   /* P^-1 A^T       (fft2: natural-order DFT)                */
+  PACG_OP_IFFT = 5,   /* conj(A) P / n  (ifft)                                   */
+  PACG_OP_UIFFT = 6   /* conj(A) / n    (uifft)                                  */
+};
+
+#define PACG_MAX_STAGES 32
+
+/* Plan description.  Axis a has length dims[a] = prod radix[a][0..nstages[a]-1];
+ * nstages[a] = 0 asks for the default schedule (radix-4 stages with one
+ * radix-2 stage on top when log2 dims[a] is odd).  Stage 0 has unit stride;
+ * its radix is the innermost digit of the digit reversal. */
+typedef struct {
+  int nd;                                 /* 1, 2 or 3 */
+  int dims[3];                            /* each a power of two, total < 2^31 */
+  int nstages[3];
+  int radix[3][PACG_MAX_STAGES];          /* entries in {2, 4, 8} */
+  int precision;                          /* PACG_PREC_F64 or PACG_PREC_F32: data arrays */
+  int spectrum_precision;                 /* storage of A^T k (always computed in FP64) */
+  int fused_axis;                         /* axis whose last pass carries the product; -1 = 0 */
+} pacg_desc;
+
+typedef struct pacg_plan pacg_plan;
+
+/* Fill a descriptor with pure radix-r schedules (r in 2, 4, 8). */
+int pacg_desc_init(pacg_desc* d, int nd, const int* dims, int r, int precision);
+
+int pacg_plan_create(pacg_plan** plan, const pacg_desc* d);
+int pacg_plan_destroy(pacg_plan* plan);
+/* Stream for every later call (default: the plan's own non-blocking stream). */
+int pacg_plan_set_stream(pacg_plan* plan, void* stream);
+void* pacg_plan_stream(const pacg_plan* plan);
+long long pacg_plan_size(const pacg_plan* plan);
+/* Digit-reversal map of the plan's schedules: rho[p] = natural index held at p. */
+int pacg_plan_rho(const pacg_plan* plan, int* rho);
+/* Number of kernel launches of one pacg_convolve and a text summary of them. */
+int pacg_plan_describe(const pacg_plan* plan, char* buf, int buflen);
+
+/* Device-buffer transforms in the plan precision.  in == out is allowed. */
+int pacg_execute(pacg_plan* plan, int op, const void* in_re, const void* in_im, void* out_re, void* out_im);
+
+/* Kernel spectrum: computes A^T k in FP64 on the device from FP64 input (host
+ * or device memory) and stores it in spectrum_precision. */
+int pacg_set_kernel(pacg_plan* plan, const double* k_re, const double* k_im, int k_on_device);
+/* Copy the stored spectrum (unscaled A^T k) to FP64 device or host memory. */
+int pacg_get_spectrum(pacg_plan* plan, double* s_re, double* s_im, int on_device);
+
+/* y = (1/n) conj(A)[(A^T k) .* (A^T x)] with the stored spectrum, device buffers. */
+int pacg_convolve(pacg_plan* plan, const void* x_re, const void* x_im, void* y_re, void* y_im);
+/* Reference semantics on device buffers: h <- A^T h (unscaled), x <- convolution. */
+int pacg_convolve_pair(pacg_plan* plan, void* x_re, void* x_im, void* h_re, void* h_im);
+/* FFT-based (permuting) convolution: h <- F h, x <- ifft(F x .* F h) (convolution.c:12-29). */
+int pacg_convolve_fft_based(pacg_plan* plan, void* x_re, void* x_im, void* h_re, void* h_im);
+
+/* Capture pacg_convolve(x -> y) on the plan stream into a HIP graph and replay it. */
+int pacg_graph_capture_convolve(pacg_plan* plan, const void* x_re, const void* x_im, void* y_re, void* y_im);
+int pacg_graph_replay(pacg_plan* plan);
+
+/* Timing aid: runs `reps` convolutions with a HIP event before every kernel
+ * launch (on the plan stream) and writes the mean duration of launch i to
+ * launch_ms[i] and its kernel name to names[i] (if names != NULL).  Returns
+ * the number of launches per convolution, or minus an error code. */
+int pacg_profile_convolve(pacg_plan* plan, const void* x_re, const void* x_im, void* y_re, void* y_im, int reps,
+                          float* launch_ms, const char** names, int max_launches);
+
+/* Free the device plans and buffers cached by the drop-in fft.h entry points. */
+void pacg_dropin_release(void);
+
+/* Wait for the plan stream. */
+int pacg_sync(pacg_plan* plan);
+
+/* Last error message of the calling thread ("" when none). */
+const char* pacg_last_error(void);
+/* Library version string. */
+const char* pacg_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PACG_H */

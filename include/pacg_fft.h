@@ -1,0 +1,189 @@
+/*
+ * pacg_fft.h -- drop-in replacement for the reference's fft.h (fft.h:17-231).
+ *
+ * Every prototype below has the reference's name, argument list and meaning,
+ * so a caller that includes fft.h and links convolution*.c / fft*.c /
+ * butterflies*.c can link libpacg.so instead.  Arrays are host memory, split
+ * complex double, row-major with the last axis contiguous; results overwrite
+ * their inputs exactly as in the reference.  The compute runs on the GPU:
+ * each call copies its arrays to the device, runs hand-written gfx950 pass
+ * kernels and copies the results back (device plans are cached per shape).
+ *
+ * Differences from the reference, all on inputs it leaves undefined:
+ *   - invalid r, non-power sizes, n >= 2^31 or NULL pointers do nothing and
+ *     set pacg_last_error() (the reference computes garbage or reads out of
+ *     bounds, or silently skips the work for r not in {2,4,8});
+ *   - the twiddle tables passed in are not read: the GPU path uses tables it
+ *     generates in FP64 on the device (same values, twiddle_factors.c:17-131);
+ *   - in 2D/3D, r = 8 runs real radix-8 stages (the reference's
+ *     convolution_2d.c:42-69 and fft_2d.c skip all butterflies for r = 8);
+ *   - rho arrays are applied as the gather x[j] <- x[rho[j]], which is the
+ *     reference's swap loop (index_reversals.c:85-94) for every rho the
+ *     reference's precompute functions return (they are involutions).
+ */
+#ifndef PACG_FFT_H
+#define PACG_FFT_H
+
+#include <complex.h>
+#include <stdbool.h>
+#include <stdlib.h>
+#include <time.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* twiddle tables -- fft.h:18-21 (host tables, kept for callers that build them) */
+void precompute_twiddles_r2(int n, double*** twiddle_re, double*** twiddle_im);
+void precompute_twiddles_r4(int n, double*** twiddle_re, double*** twiddle_im);
+void precompute_twiddles_r8(int n, double*** twiddle_re, double*** twiddle_im);
+void free_twiddles(double** twiddle_re, double** twiddle_im, int t);
+
+/* 1D butterflies A, conj(A), A^T -- fft.h:24-41 */
+void apply_butterflies_r2(double* restrict re, double* restrict im, int n, const double** twiddle_re,
+                          const double** twiddle_im);
+void apply_conjugate_butterflies_r2(double* restrict re, double* restrict im, int n, const double** twiddle_re,
+                                    const double** twiddle_im);
+void apply_transposed_butterflies_r2(double* restrict re, double* restrict im, int n, const double** twiddle_re,
+                                     const double** twiddle_im);
+void apply_butterflies_r4(double* restrict re, double* restrict im, int n, const double** twiddle_re,
+                          const double** twiddle_im);
+void apply_conjugate_butterflies_r4(double* restrict re, double* restrict im, int n, const double** twiddle_re,
+                                    const double** twiddle_im);
+void apply_transposed_butterflies_r4(double* restrict re, double* restrict im, int n, const double** twiddle_re,
+                                     const double** twiddle_im);
+void apply_butterflies_r8(double* restrict re, double* restrict im, int n, const double** twiddle_re,
+                          const double** twiddle_im);
+void apply_conjugate_butterflies_r8(double* restrict re, double* restrict im, int n, const double** twiddle_re,
+                                    const double** twiddle_im);
+void apply_transposed_butterflies_r8(double* restrict re, double* restrict im, int n, const double** twiddle_re,
+                                     const double** twiddle_im);
+
+/* 2D butterflies -- fft.h:44-67 */
+void apply_butterflies_r2_2d(double* restrict re, double* restrict im, int n1, int n2, const double** twiddle_re1,
+                             const double** twiddle_im1, const double** twiddle_re2, const double** twiddle_im2);
+void apply_conjugate_butterflies_r2_2d(double* restrict re, double* restrict im, int n1, int n2,
+                                       const double** twiddle_re1, const double** twiddle_im1,
+                                       const double** twiddle_re2, const double** twiddle_im2);
+void apply_transposed_butterflies_r2_2d(double* restrict re, double* restrict im, int n1, int n2,
+                                        const double** twiddle_re1, const double** twiddle_im1,
+                                        const double** twiddle_re2, const double** twiddle_im2);
+void apply_butterflies_r4_2d(double* restrict re, double* restrict im, int n1, int n2, const double** twiddle_re1,
+                             const double** twiddle_im1, const double** twiddle_re2, const double** twiddle_im2);
+void apply_conjugate_butterflies_r4_2d(double* restrict re, double* restrict im, int n1, int n2,
+                                       const double** twiddle_re1, const double** twiddle_im1,
+                                       const double** twiddle_re2, const double** twiddle_im2);
+void apply_transposed_butterflies_r4_2d(double* restrict re, double* restrict im, int n1, int n2,
+                                        const double** twiddle_re1, const double** twiddle_im1,
+                                        const double** twiddle_re2, const double** twiddle_im2);
+
+/* 3D butterflies -- fft.h:70-99 */
+void apply_butterflies_r2_3d(double* restrict re, double* restrict im, int n1, int n2, int n3,
+                             const double** twiddle_re1, const double** twiddle_im1, const double** twiddle_re2,
+                             const double** twiddle_im2, const double** twiddle_re3, const double** twiddle_im3);
+void apply_conjugate_butterflies_r2_3d(double* restrict re, double* restrict im, int n1, int n2, int n3,
+                                       const double** twiddle_re1, const double** twiddle_im1,
+                                       const double** twiddle_re2, const double** twiddle_im2,
+                                       const double** twiddle_re3, const double** twiddle_im3);
+void apply_transposed_butterflies_r2_3d(double* restrict re, double* restrict im, int n1, int n2, int n3,
+                                        const double** twiddle_re1, const double** twiddle_im1,
+                                        const double** twiddle_re2, const double** twiddle_im2,
+                                        const double** twiddle_re3, const double** twiddle_im3);
+void apply_butterflies_r4_3d(double* restrict re, double* restrict im, int n1, int n2, int n3,
+                             const double** twiddle_re1, const double** twiddle_im1, const double** twiddle_re2,
+                             const double** twiddle_im2, const double** twiddle_re3, const double** twiddle_im3);
+void apply_conjugate_butterflies_r4_3d(double* restrict re, double* restrict im, int n1, int n2, int n3,
+                                       const double** twiddle_re1, const double** twiddle_im1,
+                                       const double** twiddle_re2, const double** twiddle_im2,
+                                       const double** twiddle_re3, const double** twiddle_im3);
+void apply_transposed_butterflies_r4_3d(double* restrict re, double* restrict im, int n1, int n2, int n3,
+                                        const double** twiddle_re1, const double** twiddle_im1,
+                                        const double** twiddle_re2, const double** twiddle_im2,
+                                        const double** twiddle_re3, const double** twiddle_im3);
+
+/* digit reversal and permutation -- fft.h:102-118 (host integer bookkeeping) */
+int reverse_index_r2(int x, int t);
+int* precompute_index_reversal_permutation_r2(int n);
+int reverse_index_r4(int x, int t);
+int* precompute_index_reversal_permutation_r4(int n);
+int reverse_index_r8(int x, int t);
+int* precompute_index_reversal_permutation_r8(int n);
+void apply_permutation(double complex* x, int n, const int* rho);
+void apply_permutation_split(double* restrict re, double* restrict im, int n, const int* rho);
+int* precompute_index_reversal_permutation_r2_2d(int n1, int n2);
+int* precompute_index_reversal_permutation_r4_2d(int n1, int n2);
+int* precompute_index_reversal_permutation_r2_3d(int n1, int n2, int n3);
+int* precompute_index_reversal_permutation_r4_3d(int n1, int n2, int n3);
+
+/* ordered / unordered transforms -- fft.h:121-185 */
+void fft(double* restrict re, double* restrict im, int n, int r, double** twiddle_re, double** twiddle_im,
+         const int* rho);
+void fft2(double* restrict re, double* restrict im, int n, int r, double** twiddle_re, double** twiddle_im,
+          const int* rho);
+void ufft(double* restrict re, double* restrict im, int n, int r, double** twiddle_re, double** twiddle_im);
+void ifft(double* restrict re, double* restrict im, int n, int r, double** twiddle_re, double** twiddle_im,
+          const int* rho);
+void uifft(double* restrict re, double* restrict im, int n, int r, double** twiddle_re, double** twiddle_im);
+
+void fft_2d(double* restrict re, double* restrict im, int n1, int n2, int r, double** twiddle_re1,
+            double** twiddle_im1, double** twiddle_re2, double** twiddle_im2, const int* rho);
+void fft2_2d(double* restrict re, double* restrict im, int n1, int n2, int r, double** twiddle_re1,
+             double** twiddle_im1, double** twiddle_re2, double** twiddle_im2, const int* rho);
+void ufft_2d(double* restrict re, double* restrict im, int n1, int n2, int r, double** twiddle_re1,
+             double** twiddle_im1, double** twiddle_re2, double** twiddle_im2);
+void ifft_2d(double* restrict re, double* restrict im, int n1, int n2, int r, double** twiddle_re1,
+             double** twiddle_im1, double** twiddle_re2, double** twiddle_im2, const int* rho);
+void uifft_2d(double* restrict re, double* restrict im, int n1, int n2, int r, double** twiddle_re1,
+              double** twiddle_im1, double** twiddle_re2, double** twiddle_im2);
+
+void fft_3d(double* restrict re, double* restrict im, int n1, int n2, int n3, int r, double** twiddle_re1,
+            double** twiddle_im1, double** twiddle_re2, double** twiddle_im2, double** twiddle_re3,
+            double** twiddle_im3, const int* rho);
+void fft2_3d(double* restrict re, double* restrict im, int n1, int n2, int n3, int r, double** twiddle_re1,
+             double** twiddle_im1, double** twiddle_re2, double** twiddle_im2, double** twiddle_re3,
+             double** twiddle_im3, const int* rho);
+void ufft_3d(double* restrict re, double* restrict im, int n1, int n2, int n3, int r, double** twiddle_re1,
+             double** twiddle_im1, double** twiddle_re2, double** twiddle_im2, double** twiddle_re3,
+             double** twiddle_im3);
+void ifft_3d(double* restrict re, double* restrict im, int n1, int n2, int n3, int r, double** twiddle_re1,
+             double** twiddle_im1, double** twiddle_re2, double** twiddle_im2, double** twiddle_re3,
+             double** twiddle_im3, const int* rho);
+void uifft_3d(double* restrict re, double* restrict im, int n1, int n2, int n3, int r, double** twiddle_re1,
+              double** twiddle_im1, double** twiddle_re2, double** twiddle_im2, double** twiddle_re3,
+              double** twiddle_im3);
+
+/* convolutions -- fft.h:188-221 (the hot path: permutation_avoiding_convolution*) */
+void fft_based_convolution(double* restrict re, double* restrict im, int n, int r, double** twiddle_re,
+                           double** twiddle_im, const int* rho, double* restrict g_re, double* restrict g_im);
+void permutation_avoiding_convolution(double* restrict re, double* restrict im, int n, int r, double** twiddle_re,
+                                      double** twiddle_im, double* restrict Pg_re, double* restrict Pg_im);
+void fft_based_convolution_2d(double* restrict re, double* restrict im, int n1, int n2, int r,
+                              double** twiddle_re1, double** twiddle_im1, double** twiddle_re2,
+                              double** twiddle_im2, const int* rho, double* restrict g_re, double* restrict g_im);
+void permutation_avoiding_convolution_2d(double* restrict re, double* restrict im, int n1, int n2, int r,
+                                         double** twiddle_re1, double** twiddle_im1, double** twiddle_re2,
+                                         double** twiddle_im2, double* restrict Pg_re, double* restrict Pg_im);
+void fft_based_convolution_3d(double* restrict re, double* restrict im, int n1, int n2, int n3, int r,
+                              double** twiddle_re1, double** twiddle_im1, double** twiddle_re2,
+                              double** twiddle_im2, double** twiddle_re3, double** twiddle_im3, const int* rho,
+                              double* restrict g_re, double* restrict g_im);
+void permutation_avoiding_convolution_3d(double* restrict re, double* restrict im, int n1, int n2, int n3, int r,
+                                         double** twiddle_re1, double** twiddle_im1, double** twiddle_re2,
+                                         double** twiddle_im2, double** twiddle_re3, double** twiddle_im3,
+                                         double* restrict Pg_re, double* restrict Pg_im);
+
+/* host helpers -- fft.h:224-231 */
+double time_diff(struct timespec start, struct timespec end);
+double get_max_error(double* re, double* im, double* fftw_flat, int n);
+void initialize_data(double complex* x, double* fftw_in_flat, int n);
+void initialize_data_split(double* re, double* im, double* fftw_in_flat, int n);
+void initialize_filter(double* g_re, double* g_im, int n);
+bool supports_radix_4(int t2);
+bool supports_radix_8(int t2);
+void in_place_square_array_transpose_split_2d(double* re, double* im, int n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PACG_FFT_H */

@@ -1,0 +1,190 @@
+"""GPU parity: the HIP pass kernels against the oracle on the same inputs.
+
+FP64 bar: relative L2 <= 1e-12 (north_star's FP64 tolerance); the digit
+reversal / spectrum order is compared exactly (integer bookkeeping).
+FP32 bar: relative L2 <= 2e-6 against FP64 (north_star's FP32 tolerance).
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+from conftest import golden_cases, load_golden, rel_l2
+from oracle import A, CONJ, FFT, FFT2, IFFT, TRANS, UFFT, UIFFT, pure_schedule
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-12
+TOL32 = 2e-6
+
+CASES = [
+    # dims, schedules (stage 0 first)
+    ((64,), [[2] * 6]),
+    ((256,), [[4] * 4]),
+    ((512,), [[8] * 3]),
+    ((4096,), [[8] * 4]),
+    ((1 << 16,), [[2] * 16]),
+    ((1 << 16,), [[4] * 8]),
+    ((2048,), [[4] * 5 + [2]]),
+    ((256,), [[8, 8, 4]]),
+    ((256,), [[4, 8, 8]]),
+    ((32,), [[2, 2, 8]]),
+    ((8,), [[8]]),
+    ((2,), [[2]]),
+    ((1,), [[]]),
+    ((16, 64), [[4, 4], [4, 4, 4]]),
+    ((64, 4), [[4] * 3, [4]]),
+    ((128, 256), [[2] * 7, [8, 8, 4]]),
+    ((8, 16, 32), [[8], [4, 4], [2, 4, 4]]),
+    ((16, 16, 16), [[4, 4], [4, 4], [4, 4]]),
+    ((32, 32, 64), [[8, 4], [4, 8], [8, 8]]),
+    ((1, 16, 8), [[], [4, 4], [8]]),
+    ((4, 1, 8), [[4], [], [8]]),
+]
+
+
+def rnd(seed, shape):
+    g = np.random.default_rng(seed)
+    return g.uniform(-1, 1, shape) + 1j * g.uniform(-1, 1, shape)
+
+
+@pytest.mark.parametrize("dims,scheds", CASES)
+def test_convolve_fp64(gpu, oracle, dims, scheds):
+    x = rnd(1, dims)
+    k = rnd(2, dims)
+    y_ref, s_ref = oracle.pa_convolution(x, k, scheds)
+    pl = gpu.Plan(dims, schedules=scheds)
+    pl.set_kernel(k)
+    # spectrum = unscaled A^T k in the schedule's digit-reversed order
+    assert rel_l2(pl.spectrum(), s_ref) < TOL64
+    assert np.array_equal(pl.rho(), oracle.rho(list(dims), scheds).astype(np.int32))
+    dx = gpu.DeviceComplex.from_numpy(x)
+    dy = pl.alloc()
+    pl.convolve(dx, dy)
+    assert rel_l2(dy.numpy(), y_ref) < TOL64
+    pl.convolve(dx)  # in place
+    assert rel_l2(dx.numpy(), y_ref) < TOL64
+
+
+@pytest.mark.parametrize("dims,scheds", CASES)
+def test_operators_fp64(gpu, oracle, dims, scheds):
+    x = rnd(3, dims)
+    pl = gpu.Plan(dims, schedules=scheds)
+    ops = [(gpu.OP_TRANS, lambda: oracle.butterflies_nd(x, scheds, TRANS)),
+           (gpu.OP_CONJ, lambda: oracle.butterflies_nd(x, scheds, CONJ)),
+           (gpu.OP_FWD, lambda: oracle.butterflies_nd(x, scheds, A)),
+           (gpu.OP_FFT, lambda: oracle.transform(x, scheds, FFT)),
+           (gpu.OP_FFT2, lambda: oracle.transform(x, scheds, FFT2)),
+           (gpu.OP_IFFT, lambda: oracle.transform(x, scheds, IFFT)),
+           (gpu.OP_UIFFT, lambda: oracle.transform(x, scheds, UIFFT))]
+    for op, ref in ops:
+        dx = gpu.DeviceComplex.from_numpy(x)
+        dy = pl.alloc()
+        pl.execute(op, dx, dy)
+        assert rel_l2(dy.numpy(), ref()) < TOL64, op
+        pl.execute(op, dx)  # in place
+        assert rel_l2(dx.numpy(), ref()) < TOL64, op
+
+
+@pytest.mark.parametrize("dims,scheds", CASES)
+def test_pair_and_fft_based_fp64(gpu, oracle, dims, scheds):
+    x = rnd(5, dims)
+    h = rnd(6, dims)
+    pl = gpu.Plan(dims, schedules=scheds)
+    y_ref, ath_ref = oracle.pa_convolution(x, h, scheds)
+    dx, dh = gpu.DeviceComplex.from_numpy(x), gpu.DeviceComplex.from_numpy(h)
+    pl.convolve_pair(dx, dh)
+    assert rel_l2(dx.numpy(), y_ref) < TOL64
+    assert rel_l2(dh.numpy(), ath_ref) < TOL64
+    yf_ref, fh_ref = oracle.fft_based_convolution(x, h, scheds)
+    dx, dh = gpu.DeviceComplex.from_numpy(x), gpu.DeviceComplex.from_numpy(h)
+    pl.convolve_fft_based(dx, dh)
+    assert rel_l2(dx.numpy(), yf_ref) < TOL64
+    assert rel_l2(dh.numpy(), fh_ref) < TOL64
+
+
+@pytest.mark.parametrize("dims,scheds", [c for c in CASES if np.prod(c[0]) >= 8])
+@pytest.mark.parametrize("sprec", [0, 1])
+def test_convolve_fp32(gpu, oracle, dims, scheds, sprec):
+    x = rnd(7, dims)
+    k = rnd(8, dims)
+    y_ref, _ = oracle.pa_convolution(x, k, scheds)
+    pl = gpu.Plan(dims, schedules=scheds, precision=gpu.PREC_F32, spectrum_precision=sprec)
+    pl.set_kernel(k)
+    dx = gpu.DeviceComplex.from_numpy(x, np.float32)
+    pl.convolve(dx)
+    assert rel_l2(dx.numpy(), y_ref) < TOL32
+
+
+@pytest.mark.parametrize("name", golden_cases())
+def test_dropin_against_reference_golden(gpu, name):
+    g = load_golden(name)
+    dims = [int(d) for d in g["dims"]]
+    r = int(g["radix"])
+    nd = len(dims)
+    sfx = {1: "", 2: "_2d", 3: "_3d"}[nd]
+    tw = [None] * (2 * nd)
+
+    def split(z):
+        return np.ascontiguousarray(z.real.ravel()), np.ascontiguousarray(z.imag.ravel())
+
+    def join(a, b):
+        return (a + 1j * b).reshape(dims)
+
+    rho = g["rho"]
+    # permutation-avoiding convolution: x <- y, h <- A^T h
+    xr, xi = split(g["x"])
+    hr, hi = split(g["h"])
+    getattr(gpu, "permutation_avoiding_convolution" + sfx)(xr, xi, *dims, r, *tw, hr, hi)
+    assert rel_l2(join(xr, xi), g["pa_y"]) < TOL64
+    assert rel_l2(join(hr, hi), g["pa_h"]) < TOL64
+    # FFT-based convolution: x <- y, h <- F h
+    xr, xi = split(g["x"])
+    hr, hi = split(g["h"])
+    getattr(gpu, "fft_based_convolution" + sfx)(xr, xi, *dims, r, *tw, rho, hr, hi)
+    assert rel_l2(join(xr, xi), g["fb_y"]) < TOL64
+    assert rel_l2(join(hr, hi), g["fb_h"]) < TOL64
+    for nm in ("fft", "fft2", "ifft"):
+        xr, xi = split(g["x"])
+        getattr(gpu, nm + sfx)(xr, xi, *dims, r, *tw, rho)
+        assert rel_l2(join(xr, xi), g[nm]) < TOL64, nm
+    for nm in ("ufft", "uifft"):
+        xr, xi = split(g["x"])
+        getattr(gpu, nm + sfx)(xr, xi, *dims, r, *tw)
+        assert rel_l2(join(xr, xi), g[nm]) < TOL64, nm
+    if nd == 1 or r in (2, 4):
+        for kind, key in (("", "A"), ("conjugate_", "conjA"), ("transposed_", "AT")):
+            if nd > 1 and r == 8:
+                continue
+            xr, xi = split(g["x"])
+            getattr(gpu, "apply_%sbutterflies_r%d%s" % (kind, r, sfx))(xr, xi, *dims, *tw)
+            assert rel_l2(join(xr, xi), g[key]) < TOL64, kind
+
+
+def test_dropin_rejects_bad_input(gpu):
+    x = np.zeros(24)
+    y = np.zeros(24)
+    with pytest.raises(gpu.PacgError):
+        gpu.apply_transposed_butterflies_r4(x, y, 24, None, None)
+    with pytest.raises(gpu.PacgError):
+        gpu.permutation_avoiding_convolution(x[:16], y[:16], 16, 3, None, None, x[:16].copy(), y[:16].copy())
+    assert np.all(x == 0) and np.all(y == 0)
+
+
+def test_graph_replay_matches_eager(gpu, oracle):
+    dims = (64, 64)
+    scheds = [[4, 4, 4], [8, 8]]
+    x = rnd(9, dims)
+    k = rnd(10, dims)
+    y_ref, _ = oracle.pa_convolution(x, k, scheds)
+    pl = gpu.Plan(dims, schedules=scheds)
+    pl.set_kernel(k)
+    dx = gpu.DeviceComplex.from_numpy(x)
+    dy = pl.alloc()
+    pl.capture(dx, dy)
+    for it in range(3):
+        dy.upload(np.zeros(dims, complex))
+        pl.replay()
+        pl.sync()
+        assert rel_l2(dy.numpy(), y_ref) < TOL64, it
+        pl.convolve(dx, dy)  # eager work between replays
+        pl.sync()
