@@ -62,12 +62,7 @@ def time_convs(pacg, pl, dx, dy, steps, warmup):
 
 
 def profile_launches(pacg, pl, dx, dy, reps):
-    buf = (C.c_float * 64)()
-    names = (C.c_char_p * 64)()
-    nl = pacg.lib.pacg_profile_convolve(pl.ptr, dx.re.ptr, dx.im.ptr, dy.re.ptr, dy.im.ptr, reps, buf, names, 64)
-    if nl <= 0:
-        raise RuntimeError("profile failed: %s" % pacg.last_error())
-    return [(names[i].decode(), float(buf[i])) for i in range(nl)]
+    return pl.profile(dx, dy, reps)
 
 
 def alg_bytes_launch(name, n, e, s):
@@ -126,6 +121,9 @@ def main():
     ap.add_argument("--cpu-reps", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--spectrum", choices=["f32", "f64"], default="f64")
+    ap.add_argument("--profile", action="store_true",
+                    help="only run the FP32 convolution `steps` times after a copy kernel of 2 x 256 MiB "
+                         "(counter calibration), for rocprofv3")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (asked for --gpus %d)" % a.gpus,
@@ -148,6 +146,18 @@ def main():
     pl32.set_kernel(k)
     dx32 = pacg.DeviceComplex.from_numpy(x, np.float32)
     dy32 = pl32.alloc()
+    if a.profile:
+        nb = 256 << 20
+        src, dst = pacg.DeviceArray(nb // 4, np.float32), pacg.DeviceArray(nb // 4, np.float32)
+        src.upload(np.zeros(nb // 4, np.float32))
+        for _ in range(3):
+            pacg.bw_probe(2, dst, src, nb, pl32.stream)
+        for _ in range(a.steps):
+            pl32.convolve(dx32, dy32)
+        pl32.sync()
+        log("profile mode: 3 copies of %d bytes, %d convolutions" % (nb, a.steps))
+        print(json.dumps({"profile": True, "copy_bytes": nb, "steps": a.steps, "plan": pl32.describe()}))
+        return
     ms32 = time_convs(pacg, pl32, dx32, dy32, a.steps, a.warmup)
     log("fp32: %.3f ms/conv  plan: %s" % (ms32, pl32.describe()))
     prof32 = profile_launches(pacg, pl32, dx32, dy32, max(5, a.steps // 2))

@@ -67,6 +67,12 @@ def _load():
     L.pacg_graph_capture_convolve.argtypes = [_vp, _vp, _vp, _vp, _vp]
     L.pacg_graph_replay.argtypes = [_vp]
     L.pacg_sync.argtypes = [_vp]
+    L.pacg_profile_convolve.argtypes = [_vp, _vp, _vp, _vp, _vp, C.c_int, C.POINTER(C.c_float),
+                                        C.POINTER(C.c_char_p), C.c_int]
+    L.pacg_device_sync.argtypes = []
+    L.pacg_bw_probe.argtypes = [C.c_int, _vp, _vp, C.c_uint64, _vp]
+    L.pacg_dropin_release.argtypes = []
+    L.pacg_dropin_release.restype = None
     L.pacg_desc_init.argtypes = [C.POINTER(_Desc), C.c_int, _ip, C.c_int, C.c_int]
     # runtime glue (device memory, copies, events)
     L.pacg_dev_malloc.argtypes = [C.POINTER(_vp), C.c_size_t]
@@ -116,14 +122,18 @@ class DeviceArray:
         _check(lib.pacg_dev_malloc(C.byref(p), max(1, self.nbytes)), "device allocation")
         self.ptr = p.value
 
+    # Plans run on their own non-blocking streams, which the null stream does
+    # not order against: host copies wait for the whole device first.
     def upload(self, a: np.ndarray, stream=None):
         a = np.ascontiguousarray(a, dtype=self.dtype).ravel()
         assert a.size == self.n
+        _check(lib.pacg_device_sync(), "device sync")
         _check(lib.pacg_memcpy_h2d(self.ptr, a.ctypes.data, self.nbytes, stream), "H2D copy")
         _check(lib.pacg_stream_sync(stream), "sync")
 
     def download(self, stream=None) -> np.ndarray:
         out = np.empty(self.n, dtype=self.dtype)
+        _check(lib.pacg_device_sync(), "device sync")
         _check(lib.pacg_memcpy_d2h(out.ctypes.data, self.ptr, self.nbytes, stream), "D2H copy")
         _check(lib.pacg_stream_sync(stream), "sync")
         return out
@@ -259,6 +269,15 @@ class Plan:
     def sync(self):
         _check(lib.pacg_sync(self.ptr), "pacg_sync")
 
+    def profile(self, x: DeviceComplex, y: DeviceComplex, reps: int = 10):
+        """Mean duration (ms) of every kernel launch of one convolve, HIP events on the plan stream."""
+        buf = (C.c_float * 64)()
+        names = (C.c_char_p * 64)()
+        nl = lib.pacg_profile_convolve(self.ptr, x.re.ptr, x.im.ptr, y.re.ptr, y.im.ptr, int(reps), buf, names, 64)
+        if nl < 0:
+            raise PacgError("pacg_profile_convolve failed: %s" % last_error())
+        return [(names[i].decode(), float(buf[i])) for i in range(nl)]
+
     def close(self):
         if getattr(self, "ptr", None):
             lib.pacg_plan_destroy(self.ptr)
@@ -269,6 +288,11 @@ class Plan:
             self.close()
         except Exception:
             pass
+
+
+def bw_probe(mode: int, dst: DeviceArray, src: DeviceArray, nbytes: int, stream=None):
+    """HBM probe kernel: mode 0 read, 1 write, 2 copy (16-byte accesses)."""
+    _check(lib.pacg_bw_probe(int(mode), dst.ptr, src.ptr, int(nbytes), stream), "bw probe")
 
 
 class Timer:

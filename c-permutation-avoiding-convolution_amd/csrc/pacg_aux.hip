@@ -88,6 +88,27 @@ __global__ void convert_kernel(const X* __restrict__ xr, const X* __restrict__ x
   yi[p] = (Y)(xi[p] * scale);
 }
 
+// HBM probe (the GPU counterpart of the reference's memory_bw_test.c:27-67):
+// streaming read (reduced so it is not optimised away), write and copy with
+// 16-byte accesses over a grid-stride loop.
+__global__ void bw_read_kernel(const float4* __restrict__ src, uint64_t n4, float* __restrict__ sink) {
+  float acc = 0.f;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += (uint64_t)gridDim.x * blockDim.x) {
+    const float4 v = src[i];
+    acc += v.x + v.y + v.z + v.w;
+  }
+  if (acc == 1234.5f) sink[0] = acc;  // never true for the zero-filled probe buffers
+}
+__global__ void bw_write_kernel(float4* __restrict__ dst, uint64_t n4) {
+  const float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += (uint64_t)gridDim.x * blockDim.x)
+    dst[i] = v;
+}
+__global__ void bw_copy_kernel(float4* __restrict__ dst, const float4* __restrict__ src, uint64_t n4) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += (uint64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
 inline uint32_t blocks_for(uint64_t n, uint32_t t) { return (uint32_t)((n + t - 1) / t); }
 
 }  // namespace
@@ -167,6 +188,20 @@ int pacg_convert(const void* xre, const void* xim, int xprec, void* yre, void* y
   else
     hipLaunchKernelGGL((convert_kernel<float, float>), g, b, 0, s, (const float*)xre, (const float*)xim,
                        (float*)yre, (float*)yim, n, (float)scale);
+  return (int)hipGetLastError();
+}
+
+int pacg_bw_probe(int mode, void* dst, const void* src, uint64_t bytes, void* stream) {
+  const uint64_t n4 = bytes / 16;
+  if (n4 == 0) return 0;
+  const uint32_t blocks = 256 * 16;
+  hipStream_t s = (hipStream_t)stream;
+  if (mode == 0)
+    hipLaunchKernelGGL(bw_read_kernel, dim3(blocks), dim3(256), 0, s, (const float4*)src, n4, (float*)dst);
+  else if (mode == 1)
+    hipLaunchKernelGGL(bw_write_kernel, dim3(blocks), dim3(256), 0, s, (float4*)dst, n4);
+  else
+    hipLaunchKernelGGL(bw_copy_kernel, dim3(blocks), dim3(256), 0, s, (float4*)dst, (const float4*)src, n4);
   return (int)hipGetLastError();
 }
 

@@ -103,6 +103,10 @@ int pacg_pointwise(void* zre, void* zim, const void* wre, const void* wim, int p
 int pacg_convert(const void* xre, const void* xim, int xprec, void* yre, void* yim, int yprec,
                  uint64_t n, double scale, void* stream);
 
+/* HBM probe: mode 0 read `bytes` of src, 1 write `bytes` to dst, 2 copy src -> dst
+ * (16-byte accesses; the read mode writes dst[0] only in an impossible case). */
+int pacg_bw_probe(int mode, void* dst, const void* src, uint64_t bytes, void* stream);
+
 /* runtime glue implemented in HIP (so the C layer needs no HIP headers) */
 int pacg_dev_malloc(void** p, size_t bytes);
 int pacg_dev_free(void* p);

@@ -31,6 +31,9 @@
 
 #include "pacg_internal.h"
 
+// every lambda of the pass code is inlined: the register arrays must never escape
+#define PACG_INL __attribute__((always_inline))
+
 namespace pacg {
 
 // ---------------------------------------------------------------------------
@@ -136,7 +139,7 @@ struct Shape {
       if (phi(p) < plo(p)) return false;
     for (int g = 0; g < NS; ++g)
       if (lg(g) == 0) return false;
-    return E <= 16 && P <= 4096;
+    return E <= 16 && P <= 16384;
   }
 };
 
@@ -256,21 +259,21 @@ __device__ __forceinline__ void stage(T* vr, T* vi) {
   constexpr int NGRP = SH::E / R;
   constexpr int ST = 1 << lgl;
   constexpr int M = rho << lgl;  // modulus of the internal twiddle
-  sfor<NGRP>([&](auto C_) {
-    sfor<R / rho>([&](auto B_) {
+  sfor<NGRP>([&](auto C_) PACG_INL {
+    sfor<R / rho>([&](auto B_) PACG_INL {
       constexpr int bidx = decltype(B_)::value;
       constexpr int mu = bidx & (ST - 1);
       constexpr int hi = bidx >> lgl;
       constexpr int base = decltype(C_)::value * R + hi * M + mu;
       if constexpr (MODE != M_DIF) {
-        sfor<rho>([&](auto m_) {
+        sfor<rho>([&](auto m_) PACG_INL {
           constexpr int m = decltype(m_)::value;
           rot64<T, (m * mu) * (64 / M), MODE == M_DITC>(vr[base + m * ST], vi[base + m * ST]);
         });
       }
       dft<T, rho, MODE == M_DITC, base, ST>(vr, vi);
       if constexpr (MODE == M_DIF) {
-        sfor<rho>([&](auto m_) {
+        sfor<rho>([&](auto m_) PACG_INL {
           constexpr int m = decltype(m_)::value;
           rot64<T, (m * mu) * (64 / M), false>(vr[base + m * ST], vi[base + m * ST]);
         });
@@ -284,38 +287,48 @@ template <typename T, class SH, int PH, int MODE>
 __device__ __forceinline__ void phase_stages(T* vr, T* vi) {
   constexpr int lo = SH::plo(PH), hi = SH::phi(PH);
   constexpr int cnt = hi - lo + 1;
-  sfor<cnt>([&](auto i_) {
+  sfor<cnt>([&](auto i_) PACG_INL {
     constexpr int i = decltype(i_)::value;
     constexpr int g = (MODE == M_DIF) ? (hi - i) : (lo + i);
     stage<T, SH, PH, g, MODE>(vr, vi);
   });
 }
 
-// Phase-external twiddle w_{R lambda}^{mu k(r)} from the w_P table in LDS.
-template <typename T, class SH, int PH, bool CJ>
-__device__ __forceinline__ void phase_twiddle(T* vr, T* vi, const T* twr, const T* twi, int tl) {
+// Phase-external twiddle w_{R lambda}^{mu k(r)}; tw(idx, wr, wi) returns w_P^idx.
+template <typename T, class SH, int PH, bool CJ, typename TW>
+__device__ __forceinline__ void phase_twiddle_f(T* vr, T* vi, int tl, TW&& tw) {
   constexpr int lo = SH::plo(PH);
   if constexpr (lo > 0) {
     constexpr int lgR = SH::lgR(PH);
     constexpr int R = 1 << lgR;
     constexpr int lgl = SH::lglam(lo);
     constexpr int sh = SH::LGP - SH::lglam(SH::phi(PH) + 1);
-    sfor<SH::E / R>([&](auto C_) {
+    sfor<SH::E / R>([&](auto C_) PACG_INL {
       constexpr int c = decltype(C_)::value;
       const int o = tl + SH::TPL * c;
       const int mu = o & ((1 << lgl) - 1);
-      sfor<R>([&](auto r_) {
+      sfor<R>([&](auto r_) PACG_INL {
         constexpr int r = decltype(r_)::value;
         constexpr int kr = SH::rev_phase(PH, r);
         if constexpr (kr != 0) {
           const int idx = (mu * kr) << sh;
-          T wr = twr[idx], wi = twi[idx];
+          T wr, wi;
+          tw(idx, wr, wi);
           if constexpr (CJ) wi = -wi;
           cmul<T>(vr[c * R + r], vi[c * R + r], wr, wi);
         }
       });
     });
   }
+}
+
+// ... from the w_P table in LDS
+template <typename T, class SH, int PH, bool CJ>
+__device__ __forceinline__ void phase_twiddle(T* vr, T* vi, const T* twr, const T* twi, int tl) {
+  phase_twiddle_f<T, SH, PH, CJ>(vr, vi, tl, [&](int idx, T& wr, T& wi) PACG_INL {
+    wr = twr[idx];
+    wi = twi[idx];
+  });
 }
 
 // Position (within the line) of element (c, r) of thread tl in phase PH.
@@ -333,7 +346,7 @@ __device__ __forceinline__ int pos(int tl, int c, int r) {
 template <class SH>
 __device__ __forceinline__ int rev_pass(int s) {
   int out = 0;
-  sfor<SH::NS>([&](auto g_) {
+  sfor<SH::NS>([&](auto g_) PACG_INL {
     constexpr int g = decltype(g_)::value;
     const int d = (s >> SH::lglam(g)) & (SH::rho(g) - 1);
     out |= d << (SH::LGP - SH::lglam(g + 1));
@@ -422,19 +435,19 @@ __global__ void __launch_bounds__(LINES* SH::TPL)
 
   T vr[E], vi[E];
 
-  auto lds_index = [&](int s) -> int {
+  auto lds_index = [&](int s) PACG_INL -> int {
     if constexpr (LAYOUT == PACG_L_STRIDED)
       return s * LDJ + ll;
     else
       return ll * LDP + s + (s >> 5);
   };
 
-  auto load_phase = [&](auto PH_, const T* __restrict__ br, const T* __restrict__ bi) {
+  auto load_phase = [&](auto PH_, const T* __restrict__ br, const T* __restrict__ bi) PACG_INL {
     constexpr int PH = decltype(PH_)::value;
     constexpr int R = 1 << SH::lgR(PH);
-    sfor<E / R>([&](auto c_) {
+    sfor<E / R>([&](auto c_) PACG_INL {
       constexpr int c = decltype(c_)::value;
-      sfor<R>([&](auto r_) {
+      sfor<R>([&](auto r_) PACG_INL {
         constexpr int r = decltype(r_)::value;
         const int s = pos<SH, PH>(tl, c, r);
         const uint32_t off = line_base + (uint32_t)s * stride;
@@ -443,12 +456,12 @@ __global__ void __launch_bounds__(LINES* SH::TPL)
       });
     });
   };
-  auto store_phase = [&](auto PH_, T scale) {
+  auto store_phase = [&](auto PH_, T scale) PACG_INL {
     constexpr int PH = decltype(PH_)::value;
     constexpr int R = 1 << SH::lgR(PH);
-    sfor<E / R>([&](auto c_) {
+    sfor<E / R>([&](auto c_) PACG_INL {
       constexpr int c = decltype(c_)::value;
-      sfor<R>([&](auto r_) {
+      sfor<R>([&](auto r_) PACG_INL {
         constexpr int r = decltype(r_)::value;
         const int s = pos<SH, PH>(tl, c, r);
         const uint32_t off = line_base + (uint32_t)s * stride;
@@ -459,15 +472,15 @@ __global__ void __launch_bounds__(LINES* SH::TPL)
       });
     });
   };
-  auto exchange = [&](auto FROM_, auto TO_) {
+  auto exchange = [&](auto FROM_, auto TO_) PACG_INL {
     constexpr int FROM = decltype(FROM_)::value;
     constexpr int TO = decltype(TO_)::value;
     constexpr int RF = 1 << SH::lgR(FROM);
     constexpr int RT = 1 << SH::lgR(TO);
     __syncthreads();
-    sfor<E / RF>([&](auto c_) {
+    sfor<E / RF>([&](auto c_) PACG_INL {
       constexpr int c = decltype(c_)::value;
-      sfor<RF>([&](auto r_) {
+      sfor<RF>([&](auto r_) PACG_INL {
         constexpr int r = decltype(r_)::value;
         const int li = lds_index(pos<SH, FROM>(tl, c, r));
         xr[li] = vr[c * RF + r];
@@ -475,9 +488,9 @@ __global__ void __launch_bounds__(LINES* SH::TPL)
       });
     });
     __syncthreads();
-    sfor<E / RT>([&](auto c_) {
+    sfor<E / RT>([&](auto c_) PACG_INL {
       constexpr int c = decltype(c_)::value;
-      sfor<RT>([&](auto r_) {
+      sfor<RT>([&](auto r_) PACG_INL {
         constexpr int r = decltype(r_)::value;
         const int li = lds_index(pos<SH, TO>(tl, c, r));
         vr[c * RT + r] = xr[li];
@@ -486,16 +499,16 @@ __global__ void __launch_bounds__(LINES* SH::TPL)
     });
   };
   // pass-external twiddle w_K^{j' k(s)} on the phase-0 layout
-  auto ext_twiddle = [&](bool cj) {
+  auto ext_twiddle = [&](bool cj) PACG_INL {
     using V2 = typename Vec2<T>::type;
     const V2* hi = reinterpret_cast<const V2*>(a.tw_hi);
     const V2* lo = reinterpret_cast<const V2*>(a.tw_lo);
     const uint32_t jp = jpos >> a.log2_inner;
     const uint32_t mask = (1u << a.lo_bits) - 1u;
     constexpr int R0 = 1 << SH::lgR(0);
-    sfor<E / R0>([&](auto c_) {
+    sfor<E / R0>([&](auto c_) PACG_INL {
       constexpr int c = decltype(c_)::value;
-      sfor<R0>([&](auto r_) {
+      sfor<R0>([&](auto r_) PACG_INL {
         constexpr int r = decltype(r_)::value;
         const int s = pos<SH, 0>(tl, c, r);
         const uint32_t e = (jp * (uint32_t)rev_pass<SH>(s)) << a.ext_shift;
@@ -512,7 +525,7 @@ __global__ void __launch_bounds__(LINES* SH::TPL)
   if constexpr (KIND == PACG_K_DIF || KIND == PACG_K_FUSED) {
     load_phase(std::integral_constant<int, NP - 1>{}, inr, ini);
     __syncthreads();  // twiddle table visible
-    sfor<NP>([&](auto i_) {
+    sfor<NP>([&](auto i_) PACG_INL {
       constexpr int PH = NP - 1 - decltype(i_)::value;
       if constexpr (PH < NP - 1) exchange(std::integral_constant<int, PH + 1>{}, std::integral_constant<int, PH>{});
       phase_stages<T, SH, PH, M_DIF>(vr, vi);
@@ -527,9 +540,9 @@ __global__ void __launch_bounds__(LINES* SH::TPL)
       const TS* __restrict__ hi = reinterpret_cast<const TS*>(a.spec_im);
       constexpr int R0 = 1 << SH::lgR(0);
       const T sc = (T)a.scale;
-      sfor<E / R0>([&](auto c_) {
+      sfor<E / R0>([&](auto c_) PACG_INL {
         constexpr int c = decltype(c_)::value;
-        sfor<R0>([&](auto r_) {
+        sfor<R0>([&](auto r_) PACG_INL {
           constexpr int r = decltype(r_)::value;
           const int s = pos<SH, 0>(tl, c, r);
           const uint32_t off = line_base + (uint32_t)s * stride;
@@ -540,7 +553,7 @@ __global__ void __launch_bounds__(LINES* SH::TPL)
           vi[c * R0 + r] *= sc;
         });
       });
-      sfor<NP>([&](auto i_) {
+      sfor<NP>([&](auto i_) PACG_INL {
         constexpr int PH = decltype(i_)::value;
         if constexpr (PH > 0) exchange(std::integral_constant<int, PH - 1>{}, std::integral_constant<int, PH>{});
         phase_twiddle<T, SH, PH, true>(vr, vi, twr, twi, tl);
@@ -553,7 +566,7 @@ __global__ void __launch_bounds__(LINES* SH::TPL)
     load_phase(std::integral_constant<int, 0>{}, inr, ini);
     __syncthreads();
     if (a.ext) ext_twiddle(MODE == M_DITC);
-    sfor<NP>([&](auto i_) {
+    sfor<NP>([&](auto i_) PACG_INL {
       constexpr int PH = decltype(i_)::value;
       if constexpr (PH > 0) exchange(std::integral_constant<int, PH - 1>{}, std::integral_constant<int, PH>{});
       phase_twiddle<T, SH, PH, MODE == M_DITC>(vr, vi, twr, twi, tl);
