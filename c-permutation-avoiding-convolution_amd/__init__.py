@@ -70,6 +70,7 @@ def _load():
     L.pacg_profile_convolve.argtypes = [_vp, _vp, _vp, _vp, _vp, C.c_int, C.POINTER(C.c_float),
                                         C.POINTER(C.c_char_p), C.c_int]
     L.pacg_device_sync.argtypes = []
+    L.pacg_plan_autotune.argtypes = [_vp, _vp, _vp, _vp, _vp, C.c_int]
     L.pacg_bw_probe.argtypes = [C.c_int, _vp, _vp, C.c_uint64, _vp]
     L.pacg_dropin_release.argtypes = []
     L.pacg_dropin_release.restype = None
@@ -138,10 +139,21 @@ class DeviceArray:
         _check(lib.pacg_stream_sync(stream), "sync")
         return out
 
+    @classmethod
+    def view(cls, base: "DeviceArray", offset: int, n: int) -> "DeviceArray":
+        """A non-owning window of ``n`` elements starting ``offset`` elements into ``base``."""
+        v = cls.__new__(cls)
+        v.n, v.dtype = int(n), base.dtype
+        v.nbytes = v.n * v.dtype.itemsize
+        assert offset + n <= base.n
+        v.ptr = base.ptr + int(offset) * base.dtype.itemsize
+        v._owner = base
+        return v
+
     def free(self):
-        if self.ptr:
+        if self.ptr and not hasattr(self, "_owner"):
             lib.pacg_dev_free(self.ptr)
-            self.ptr = None
+        self.ptr = None
 
     def __del__(self):
         try:
@@ -159,6 +171,18 @@ class DeviceComplex:
         self.re = DeviceArray(n, dtype)
         self.im = DeviceArray(n, dtype)
         self.dtype = np.dtype(dtype)
+
+    @classmethod
+    def shifted(cls, shape, dtype=np.float64, offset: int = 1):
+        """Arrays that start ``offset`` elements into their allocation (unaligned base pointers)."""
+        d = cls.__new__(cls)
+        d.shape = tuple(int(s) for s in shape)
+        n = int(np.prod(d.shape))
+        d._bases = (DeviceArray(n + offset, dtype), DeviceArray(n + offset, dtype))
+        d.re = DeviceArray.view(d._bases[0], offset, n)
+        d.im = DeviceArray.view(d._bases[1], offset, n)
+        d.dtype = np.dtype(dtype)
+        return d
 
     @classmethod
     def from_numpy(cls, z: np.ndarray, dtype=np.float64):
@@ -268,6 +292,13 @@ class Plan:
 
     def sync(self):
         _check(lib.pacg_sync(self.ptr), "pacg_sync")
+
+    def autotune(self, x: DeviceComplex, y: DeviceComplex, reps: int = 5) -> int:
+        """Pick the fastest compiled tile shape for each launch of convolve (x -> y)."""
+        rc = lib.pacg_plan_autotune(self.ptr, x.re.ptr, x.im.ptr, y.re.ptr, y.im.ptr, int(reps))
+        if rc < 0:
+            raise PacgError("pacg_plan_autotune failed: %s" % last_error())
+        return rc
 
     def profile(self, x: DeviceComplex, y: DeviceComplex, reps: int = 10):
         """Mean duration (ms) of every kernel launch of one convolve, HIP events on the plan stream."""

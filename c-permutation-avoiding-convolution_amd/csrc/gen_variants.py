@@ -76,7 +76,26 @@ def geometry(shape, prec):
     return P, E, tpl, strided_lines, nl
 
 
-NCHUNK = 6
+def tile_options(shape, prec, layout):
+    """Lines per workgroup to compile: the default first, then the autotuner's alternatives."""
+    P, E, tpl, sl, nl = geometry(shape, prec)
+    elem = 4 if prec == "float" else 8
+    if layout == "PACG_L_STRIDED":
+        opts = [sl]
+        if P >= 64:
+            for alt in (sl // 2, sl // 4):
+                if alt >= 4 and alt * tpl >= 64:
+                    opts.append(alt)
+    else:
+        opts = [nl]
+        if P >= 64:
+            for alt in (nl // 2, nl * 2):
+                if 1 <= alt and alt * tpl <= 1024 and alt * tpl >= 64 and alt * (P + P // 32 + 1) * 2 * elem <= 80 * 1024:
+                    opts.append(alt)
+    return opts
+
+
+NCHUNK = 8
 
 
 def emit(prec: str, outdir: str):
@@ -93,15 +112,16 @@ def emit(prec: str, outdir: str):
         sname = "x".join(str(r) for r in shape)
         for kind in KINDS:
             for layout in LAYOUTS:
-                ln = sl if layout == "PACG_L_STRIDED" else nl
                 specs = spec_types if kind == "PACG_K_FUSED" else [T]
-                for ts in specs:
-                    fn = "launch_pass<%s, %s, %s, %s, %s, %d>" % (T, ts, shname, kind, layout, ln)
-                    sp = "PACG_F64" if ts == "double" else "PACG_F32"
-                    nm = "%s_%s_%s_%s_%s" % (tag, kind[7:].lower(), "s" if layout == "PACG_L_STRIDED" else "c", sname,
-                                             "h64" if (kind == "PACG_K_FUSED" and ts == "double") else "h")
-                    rows.append('  {%s, %s, %s, %s, %d, 0x%Xu, 0x%Xu, %d, %d, %d, &%s, "%s"},'
-                                % (pconst, sp, kind, layout, len(shape), sc, pc, P, ln, ln * tpl, fn, nm))
+                for rank, ln in enumerate(tile_options(shape, prec, layout)):
+                    for ts in specs:
+                        fn = "launch_pass<%s, %s, %s, %s, %s, %d>" % (T, ts, shname, kind, layout, ln)
+                        sp = "PACG_F64" if ts == "double" else "PACG_F32"
+                        nm = "%s_%s_%s_%s_%s_l%d" % (tag, kind[7:].lower(), "s" if layout == "PACG_L_STRIDED" else "c",
+                                                     sname, "h64" if (kind == "PACG_K_FUSED" and ts == "double") else "h",
+                                                     ln)
+                        rows.append('  {%s, %s, %s, %s, %d, 0x%Xu, 0x%Xu, %d, %d, %d, %d, &%s, "%s"},'
+                                    % (pconst, sp, kind, layout, len(shape), sc, pc, P, ln, ln * tpl, rank, fn, nm))
     names = []
     for k in range(NCHUNK):
         part = rows[k::NCHUNK]

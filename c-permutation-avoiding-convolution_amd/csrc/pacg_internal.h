@@ -47,6 +47,7 @@ typedef struct {
   uint32_t ext_shift;    /* log2(N_axis / K), K = P * L */
   uint32_t lo_bits;
   uint32_t ext;          /* 1: apply the pass-external twiddle w_K^{j' k} */
+  uint32_t vec16;        /* 1: in/out base pointers are 16-byte aligned (vector access allowed) */
   double scale;          /* fused: factor applied with the spectrum (1/n) */
   double out_scale;      /* factor applied at the final store (1 or 1/n) */
 } pacg_pass_args_t;
@@ -65,6 +66,7 @@ typedef struct {
   int P;
   int lines;             /* lines per workgroup (TJ or NL) */
   int threads;
+  int rank;              /* 0: the default tile for its key; > 0: autotuner alternatives */
   pacg_launch_fn launch;
   const char* name;
 } pacg_variant_t;

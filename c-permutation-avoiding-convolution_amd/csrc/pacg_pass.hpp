@@ -372,6 +372,30 @@ struct Vec2<double> {
   using type = double2;
 };
 
+// 16-byte vectors: 4 floats or 2 doubles
+template <typename T>
+struct Vec4;
+template <>
+struct Vec4<float> {
+  using type = float4;
+  static __device__ __forceinline__ void split(const float4& v, float* o) {
+    o[0] = v.x;
+    o[1] = v.y;
+    o[2] = v.z;
+    o[3] = v.w;
+  }
+  static __device__ __forceinline__ float4 join(const float* o) { return make_float4(o[0], o[1], o[2], o[3]); }
+};
+template <>
+struct Vec4<double> {
+  using type = double2;
+  static __device__ __forceinline__ void split(const double2& v, double* o) {
+    o[0] = v.x;
+    o[1] = v.y;
+  }
+  static __device__ __forceinline__ double2 join(const double* o) { return make_double2(o[0], o[1]); }
+};
+
 // ---------------------------------------------------------------------------
 // the pass kernel
 // ---------------------------------------------------------------------------
@@ -442,9 +466,30 @@ __global__ void __launch_bounds__(LINES* SH::TPL)
       return ll * LDP + s + (s >> 5);
   };
 
+  // Phase 0 of a contiguous line holds R consecutive points per group:
+  // move them as 16-byte vectors (VW elements) when the base pointers allow.
+  constexpr int VW = 16 / (int)sizeof(T);
+  constexpr bool VEC0 = (LAYOUT == PACG_L_CONTIG) && ((1 << SH::lgR(0)) % VW == 0);
+  using V4 = typename Vec4<T>::type;
   auto load_phase = [&](auto PH_, const T* __restrict__ br, const T* __restrict__ bi) PACG_INL {
     constexpr int PH = decltype(PH_)::value;
     constexpr int R = 1 << SH::lgR(PH);
+    if constexpr (PH == 0 && VEC0) {
+      if (a.vec16) {
+        sfor<E / R>([&](auto c_) PACG_INL {
+          constexpr int c = decltype(c_)::value;
+          const uint32_t off = line_base + (uint32_t)pos<SH, 0>(tl, c, 0);
+          sfor<R / VW>([&](auto v_) PACG_INL {
+            constexpr int v = decltype(v_)::value;
+            V4 xr4 = valid ? reinterpret_cast<const V4*>(br + off)[v] : V4{};
+            V4 xi4 = valid ? reinterpret_cast<const V4*>(bi + off)[v] : V4{};
+            Vec4<T>::split(xr4, &vr[c * R + v * VW]);
+            Vec4<T>::split(xi4, &vi[c * R + v * VW]);
+          });
+        });
+        return;
+      }
+    }
     sfor<E / R>([&](auto c_) PACG_INL {
       constexpr int c = decltype(c_)::value;
       sfor<R>([&](auto r_) PACG_INL {
@@ -459,6 +504,28 @@ __global__ void __launch_bounds__(LINES* SH::TPL)
   auto store_phase = [&](auto PH_, T scale) PACG_INL {
     constexpr int PH = decltype(PH_)::value;
     constexpr int R = 1 << SH::lgR(PH);
+    if constexpr (PH == 0 && VEC0) {
+      if (a.vec16) {
+        sfor<E / R>([&](auto c_) PACG_INL {
+          constexpr int c = decltype(c_)::value;
+          const uint32_t off = line_base + (uint32_t)pos<SH, 0>(tl, c, 0);
+          sfor<R / VW>([&](auto v_) PACG_INL {
+            constexpr int v = decltype(v_)::value;
+            T tr[VW], ti[VW];
+#pragma unroll
+            for (int u = 0; u < VW; ++u) {
+              tr[u] = vr[c * R + v * VW + u] * scale;
+              ti[u] = vi[c * R + v * VW + u] * scale;
+            }
+            if (valid) {
+              reinterpret_cast<V4*>(outr + off)[v] = Vec4<T>::join(tr);
+              reinterpret_cast<V4*>(outi + off)[v] = Vec4<T>::join(ti);
+            }
+          });
+        });
+        return;
+      }
+    }
     sfor<E / R>([&](auto c_) PACG_INL {
       constexpr int c = decltype(c_)::value;
       sfor<R>([&](auto r_) PACG_INL {
@@ -522,6 +589,25 @@ __global__ void __launch_bounds__(LINES* SH::TPL)
     });
   };
 
+  // fused pass: the spectrum at the phase-0 positions, loaded up front so its
+  // latency hides behind the forward phases
+  constexpr int R0 = 1 << SH::lgR(0);
+  TS hr_[KIND == PACG_K_FUSED ? E : 1], hi_[KIND == PACG_K_FUSED ? E : 1];
+  if constexpr (KIND == PACG_K_FUSED) {
+    const TS* __restrict__ hr = reinterpret_cast<const TS*>(a.spec_re);
+    const TS* __restrict__ hi = reinterpret_cast<const TS*>(a.spec_im);
+    sfor<E / R0>([&](auto c_) PACG_INL {
+      constexpr int c = decltype(c_)::value;
+      sfor<R0>([&](auto r_) PACG_INL {
+        constexpr int r = decltype(r_)::value;
+        const int s = pos<SH, 0>(tl, c, r);
+        const uint32_t off = line_base + (uint32_t)s * stride;
+        hr_[c * R0 + r] = valid ? hr[off] : TS(0);
+        hi_[c * R0 + r] = valid ? hi[off] : TS(0);
+      });
+    });
+  }
+
   if constexpr (KIND == PACG_K_DIF || KIND == PACG_K_FUSED) {
     load_phase(std::integral_constant<int, NP - 1>{}, inr, ini);
     __syncthreads();  // twiddle table visible
@@ -536,23 +622,13 @@ __global__ void __launch_bounds__(LINES* SH::TPL)
       store_phase(std::integral_constant<int, 0>{}, (T)a.out_scale);
     } else {
       // spectral product at the same positions: x <- (x * h) * scale
-      const TS* __restrict__ hr = reinterpret_cast<const TS*>(a.spec_re);
-      const TS* __restrict__ hi = reinterpret_cast<const TS*>(a.spec_im);
-      constexpr int R0 = 1 << SH::lgR(0);
       const T sc = (T)a.scale;
-      sfor<E / R0>([&](auto c_) PACG_INL {
-        constexpr int c = decltype(c_)::value;
-        sfor<R0>([&](auto r_) PACG_INL {
-          constexpr int r = decltype(r_)::value;
-          const int s = pos<SH, 0>(tl, c, r);
-          const uint32_t off = line_base + (uint32_t)s * stride;
-          T wr = valid ? (T)hr[off] : T(0);
-          T wi = valid ? (T)hi[off] : T(0);
-          cmul<T>(vr[c * R0 + r], vi[c * R0 + r], wr, wi);
-          vr[c * R0 + r] *= sc;
-          vi[c * R0 + r] *= sc;
-        });
-      });
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        cmul<T>(vr[e], vi[e], (T)hr_[e], (T)hi_[e]);
+        vr[e] *= sc;
+        vi[e] *= sc;
+      }
       sfor<NP>([&](auto i_) PACG_INL {
         constexpr int PH = decltype(i_)::value;
         if constexpr (PH > 0) exchange(std::integral_constant<int, PH - 1>{}, std::integral_constant<int, PH>{});

@@ -112,8 +112,8 @@ static const pacg_variant_t* find_variant(int prec, int spec_prec, int kind, int
     const pacg_variant_t* tab = pacg_variant_tables[t];
     for (int i = 0; i < pacg_variant_counts[t]; ++i) {
       const pacg_variant_t* v = &tab[i];
-      if (v->prec == prec && v->kind == kind && v->layout == layout && v->nstages == ns && v->stage_code == code &&
-          (kind != PACG_K_FUSED || v->spec_prec == spec_prec))
+      if (v->rank == 0 && v->prec == prec && v->kind == kind && v->layout == layout && v->nstages == ns &&
+          v->stage_code == code && (kind != PACG_K_FUSED || v->spec_prec == spec_prec))
         return v;
     }
   }
@@ -428,6 +428,7 @@ static int launch_step(pacg_plan* pl, step_t st, int prec, int kind, int sprec, 
   A.lo_bits = (uint32_t)ax->lo_bits;
   A.scale = scale;
   A.out_scale = out_scale;
+  A.vec16 = (((uintptr_t)ir | (uintptr_t)ii | (uintptr_t)orr | (uintptr_t)oi) & 15u) == 0;
   int rc = v->launch(&A, pl->stream);
   if (rc) return pacg_fail(PACG_ERR_HIP, "launch of %s failed: %s", v->name, pacg_hip_error_string(rc));
   return PACG_OK;
@@ -498,6 +499,67 @@ static int run_conv_ev(pacg_plan* pl, int prec, const void* xr, const void* xi, 
 static int run_conv(pacg_plan* pl, int prec, const void* xr, const void* xi, void* yr, void* yi, const void* sr,
                     const void* si, int sprec) {
   return run_conv_ev(pl, prec, xr, xi, yr, yi, sr, si, sprec, NULL);
+}
+
+/* Autotuner: for every launch of one convolution, time each compiled tile
+ * shape of its pass (same stages, kind, layout and precisions) on the given
+ * buffers, out of place x -> y, and keep the fastest.  Returns the number of
+ * launches whose kernel changed, or minus an error code. */
+int pacg_plan_autotune(pacg_plan* pl, const void* xr, const void* xi, void* yr, void* yi, int reps) {
+  if (!pl || !xr || !xi || !yr || !yi || reps < 1) return -pacg_fail(PACG_ERR_ARG, "pacg_plan_autotune: bad argument");
+  if (!pl->has_spec) return -pacg_fail(PACG_ERR_STATE, "no kernel spectrum set");
+  if (pl->nsteps == 0) return 0;
+  int prec = pidx(pl), sp = pl->d.spectrum_precision, changed = 0;
+  void *e0 = NULL, *e1 = NULL;
+  if (pacg_event_create(&e0) || pacg_event_create(&e1)) return -pacg_fail(PACG_ERR_HIP, "event creation failed");
+  int nl = 2 * pl->nsteps - 1, rc = PACG_OK;
+  for (int i = 0; i < nl && !rc; ++i) {
+    step_t st;
+    int kind;
+    conv_launch(pl, i, &st, &kind);
+    pass_t* ps = &pl->ax[st.axis].pass[st.pass];
+    int vsp = kind == PACG_K_FUSED ? sp : prec;
+    const pacg_variant_t* cur = ps->v[prec][kind][vsp];
+    if (!cur) continue;
+    const pacg_variant_t* best = cur;
+    float best_ms = 1e30f;
+    for (int t = 0; t < pacg_num_variant_tables && !rc; ++t) {
+      for (int j = 0; j < pacg_variant_counts[t] && !rc; ++j) {
+        const pacg_variant_t* v = &pacg_variant_tables[t][j];
+        if (v->prec != cur->prec || v->kind != cur->kind || v->layout != cur->layout || v->nstages != cur->nstages ||
+            v->stage_code != cur->stage_code || v->spec_prec != cur->spec_prec)
+          continue;
+        ps->v[prec][kind][vsp] = v;
+        const void* s_re = kind == PACG_K_FUSED ? pl->spec_re : NULL;
+        const void* s_im = kind == PACG_K_FUSED ? pl->spec_im : NULL;
+        double scale = kind == PACG_K_FUSED ? 1.0 / (double)pl->n : 1.0;
+        if ((rc = launch_step(pl, st, prec, kind, vsp, xr, xi, yr, yi, s_re, s_im, scale, 1.0))) break;
+        if (pacg_event_record(e0, pl->stream)) rc = pacg_fail(PACG_ERR_HIP, "event record failed");
+        for (int r = 0; r < reps && !rc; ++r)
+          rc = launch_step(pl, st, prec, kind, vsp, xr, xi, yr, yi, s_re, s_im, scale, 1.0);
+        if (rc) break;
+        if (pacg_event_record(e1, pl->stream) || pacg_stream_sync(pl->stream)) {
+          rc = pacg_fail(PACG_ERR_HIP, "autotune timing failed");
+          break;
+        }
+        float ms = 0.0f;
+        pacg_event_elapsed_ms(&ms, e0, e1);
+        if (ms < best_ms) {
+          best_ms = ms;
+          best = v;
+        }
+      }
+    }
+    ps->v[prec][kind][vsp] = best;
+    if (best != cur) ++changed;
+  }
+  pacg_event_destroy(e0);
+  pacg_event_destroy(e1);
+  if (pl->graph) {
+    pacg_graph_destroy(pl->graph);
+    pl->graph = NULL;
+  }
+  return rc ? -rc : changed;
 }
 
 int pacg_profile_convolve(pacg_plan* pl, const void* xr, const void* xi, void* yr, void* yi, int reps,

@@ -102,6 +102,12 @@ int pacg_convolve_fft_based(pacg_plan* plan, void* x_re, void* x_im, void* h_re,
 int pacg_graph_capture_convolve(pacg_plan* plan, const void* x_re, const void* x_im, void* y_re, void* y_im);
 int pacg_graph_replay(pacg_plan* plan);
 
+/* Autotuner: times every compiled tile shape of each launch of pacg_convolve
+ * on these device buffers (x -> y, `reps` launches each) and keeps the
+ * fastest per launch.  Returns how many launches changed kernel, or minus an
+ * error code.  Drops a captured graph (capture again afterwards). */
+int pacg_plan_autotune(pacg_plan* plan, const void* x_re, const void* x_im, void* y_re, void* y_im, int reps);
+
 /* Timing aid: runs `reps` convolutions with a HIP event before every kernel
  * launch (on the plan stream) and writes the mean duration of launch i to
  * launch_ms[i] and its kernel name to names[i] (if names != NULL).  Returns

@@ -188,3 +188,42 @@ def test_graph_replay_matches_eager(gpu, oracle):
         assert rel_l2(dy.numpy(), y_ref) < TOL64, it
         pl.convolve(dx, dy)  # eager work between replays
         pl.sync()
+
+
+@pytest.mark.parametrize("prec,tol", [(0, TOL64), (1, TOL32)])
+def test_autotuned_plan_keeps_parity(gpu, oracle, prec, tol):
+    dims = (64, 32, 256)
+    scheds = [[8, 8], [4, 8], [8, 8, 4]]
+    x = rnd(11, dims)
+    k = rnd(12, dims)
+    y_ref, _ = oracle.pa_convolution(x, k, scheds)
+    pl = gpu.Plan(dims, schedules=scheds, precision=prec)
+    pl.set_kernel(k)
+    dt = np.float64 if prec == 0 else np.float32
+    dx = gpu.DeviceComplex.from_numpy(x, dt)
+    dy = pl.alloc()
+    pl.autotune(dx, dy, 2)
+    pl.convolve(dx, dy)
+    assert rel_l2(dy.numpy(), y_ref) < tol
+    names = [nm for nm, _ in pl.profile(dx, dy, 2)]
+    assert len(names) == pl.launches_per_convolve() == 5
+
+
+@pytest.mark.parametrize("prec,tol", [(0, TOL64), (1, TOL32)])
+def test_unaligned_buffers_take_the_scalar_path(gpu, oracle, prec, tol):
+    dims = (16, 256)
+    scheds = [[4, 4], [8, 8, 4]]
+    x = rnd(13, dims)
+    k = rnd(14, dims)
+    y_ref, _ = oracle.pa_convolution(x, k, scheds)
+    pl = gpu.Plan(dims, schedules=scheds, precision=prec)
+    pl.set_kernel(k)
+    dt = np.float64 if prec == 0 else np.float32
+    dx = gpu.DeviceComplex.shifted(dims, dt, 1)
+    dx.upload(x)
+    dy = gpu.DeviceComplex.shifted(dims, dt, 3)
+    pl.convolve(dx, dy)
+    assert rel_l2(dy.numpy(), y_ref) < tol
+    for op, kind in ((gpu.OP_TRANS, TRANS), (gpu.OP_CONJ, CONJ), (gpu.OP_FWD, A)):
+        pl.execute(op, dx, dy)
+        assert rel_l2(dy.numpy(), oracle.butterflies_nd(x, scheds, kind)) < tol
