@@ -113,6 +113,62 @@ def cpu_baseline(x, k, reps):
             "n": int(n)}
 
 
+def bench_batched(pacg, synth, steps, warmup):
+    """4096 FP32 pairs of lengths 2^U{8..14} (seed 4) in one persistent, graph-captured launch,
+    against per-length plain FP64 launches (one batch-axis plan per length)."""
+    xs, ks = synth.batched_inputs(4, 4096)
+    ms = np.array([int(np.log2(len(x))) for x in xs], dtype=np.int32)
+    b = pacg.Batch(ms)
+    dx = pacg.DeviceComplex.from_numpy(b.pack(xs), np.float32)
+    dk = pacg.DeviceComplex.from_numpy(b.pack(ks), np.float32)
+    dy = b.alloc()
+    changed = b.autotune(dx, dk, dy, 3)
+    b.capture(dx, dk, dy)
+    for _ in range(warmup):
+        b.replay()
+    b.sync()
+    t = pacg.Timer(b.stream)
+    t.start()
+    for _ in range(steps):
+        b.replay()
+    ms_batch = t.stop() / steps
+    ys = b.unpack(dy.numpy())
+    # per-length plain FP64 launches
+    ms64 = 0.0
+    num = den = 0.0
+    maxerr = 0.0
+    for m in range(8, 15):
+        idx = np.nonzero(ms == m)[0]
+        if idx.size == 0:
+            continue
+        X = np.stack([xs[i] for i in idx])
+        K = np.stack([ks[i] for i in idx])
+        pl = pacg.Plan(X.shape, schedules=["batch", None], precision=pacg.PREC_F64)
+        pl.set_kernel(K)
+        ddx = pacg.DeviceComplex.from_numpy(X)
+        ddy = pl.alloc()
+        ms64 += time_convs(pacg, pl, ddx, ddy, max(3, steps // 4), 1)
+        Y = ddy.numpy()
+        for j, i in enumerate(idx):
+            d = ys[i] - Y[j]
+            num += float(np.vdot(d, d).real)
+            den += float(np.vdot(Y[j], Y[j]).real)
+            maxerr = max(maxerr, float(np.abs(d).max()))
+        del pl, ddx, ddy
+    total = int(b.total)
+    b_alg = 24 * total
+    flops = float(sum(3 * 5 * (1 << int(m)) * int(m) for m in ms))
+    return {"workload": "4096 FP32 pairs, lengths 2^U{8..14} (seed 4), one persistent hipGraph-captured launch",
+            "points": total, "ms_per_batch": ms_batch, "conv_per_s": 4096 * 1000.0 / ms_batch,
+            "alg_bytes": b_alg, "achieved_GBs": b_alg / (ms_batch * 1e-3) / 1e9,
+            "frac": b_alg / (ms_batch * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "gflops": flops / (ms_batch * 1e-3) / 1e9,
+            "variants": {int(m): pacg.Batch.schedule(int(m), b.variant(int(m))) for m in range(8, 15)},
+            "autotuned_lengths": changed,
+            "fp64_per_length_ms": ms64, "speedup_vs_fp64_per_length": ms64 / ms_batch,
+            "rel_l2_vs_fp64": (num / den) ** 0.5 if den > 0 else 0.0, "max_abs_vs_fp64": maxerr}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -123,6 +179,7 @@ def main():
     ap.add_argument("--spectrum", choices=["f32", "f64"], default="f32",
                     help="storage of the FP64-computed spectrum on the headline FP32 path")
     ap.add_argument("--no-tune", action="store_true", help="skip the tile autotuner")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched uneven-length workload")
     ap.add_argument("--profile", action="store_true",
                     help="only run the FP32 convolution `steps` times after a copy kernel of 2 x 256 MiB "
                          "(counter calibration), for rocprofv3")
@@ -241,6 +298,10 @@ def main():
         "launches_fp32": [{"kernel": nm, "ms": t} for nm, t in prof32],
         "launches_fp64": [{"kernel": nm, "ms": t} for nm, t in prof64],
     }
+    if not a.no_batched:
+        log("batched workload ...")
+        out["batched"] = bench_batched(pacg, synth, a.steps, a.warmup)
+        log("batched: %.3f ms per batch of 4096" % out["batched"]["ms_per_batch"])
     if not a.no_cpu:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(x, k, a.cpu_reps)

@@ -70,6 +70,24 @@ def _load():
     L.pacg_profile_convolve.argtypes = [_vp, _vp, _vp, _vp, _vp, C.c_int, C.POINTER(C.c_float),
                                         C.POINTER(C.c_char_p), C.c_int]
     L.pacg_device_sync.argtypes = []
+    L.pacg_batch_create.argtypes = [C.POINTER(_vp), C.c_int, _ip]
+    L.pacg_batch_destroy.argtypes = [_vp]
+    L.pacg_batch_total.argtypes = [_vp]
+    L.pacg_batch_total.restype = C.c_longlong
+    L.pacg_batch_offset.argtypes = [_vp, C.c_int]
+    L.pacg_batch_offset.restype = C.c_longlong
+    L.pacg_batch_stream.argtypes = [_vp]
+    L.pacg_batch_stream.restype = _vp
+    L.pacg_batch_items.argtypes = [_vp]
+    for nm in ("pacg_batch_convolve", "pacg_batch_capture"):
+        getattr(L, nm).argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, _vp]
+    L.pacg_batch_autotune.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, _vp, C.c_int]
+    L.pacg_batch_set_variant.argtypes = [_vp, C.c_int, C.c_int]
+    L.pacg_batch_get_variant.argtypes = [_vp, C.c_int]
+    L.pacg_batch_num_variants.argtypes = [C.c_int]
+    L.pacg_batch_schedule.argtypes = [C.c_int, C.c_int, _ip, C.c_int]
+    L.pacg_batch_replay.argtypes = [_vp]
+    L.pacg_batch_sync.argtypes = [_vp]
     L.pacg_plan_autotune.argtypes = [_vp, _vp, _vp, _vp, _vp, C.c_int]
     L.pacg_bw_probe.argtypes = [C.c_int, _vp, _vp, C.c_uint64, _vp]
     L.pacg_dropin_release.argtypes = []
@@ -220,7 +238,9 @@ class Plan:
             for a, n in enumerate(dims):
                 d.dims[a] = n
                 sc = (schedules or [None] * len(dims))[a]
-                if sc:
+                if isinstance(sc, str) and sc == "batch":
+                    d.nstages[a] = -1  # batch axis: rows convolved independently
+                elif sc:
                     d.nstages[a] = len(sc)
                     for q, r in enumerate(sc):
                         d.radix[a][q] = int(r)
@@ -324,6 +344,82 @@ class Plan:
 def bw_probe(mode: int, dst: DeviceArray, src: DeviceArray, nbytes: int, stream=None):
     """HBM probe kernel: mode 0 read, 1 write, 2 copy (16-byte accesses)."""
     _check(lib.pacg_bw_probe(int(mode), dst.ptr, src.ptr, int(nbytes), stream), "bw probe")
+
+
+class Batch:
+    """Many FP32 convolutions of lengths 2^8..2^14 in one persistent launch (pacg_batch_*)."""
+
+    def __init__(self, log2_lengths):
+        m = np.ascontiguousarray(log2_lengths, dtype=np.int32)
+        p = _vp()
+        _check(lib.pacg_batch_create(C.byref(p), int(m.size), m.ctypes.data_as(_ip)), "pacg_batch_create")
+        self.ptr = p.value
+        self.log2n = m.copy()
+        self.count = int(m.size)
+        self.total = int(lib.pacg_batch_total(self.ptr))
+        self.offsets = np.array([lib.pacg_batch_offset(self.ptr, i) for i in range(self.count)], dtype=np.int64)
+
+    @property
+    def stream(self):
+        return lib.pacg_batch_stream(self.ptr)
+
+    def alloc(self) -> DeviceComplex:
+        return DeviceComplex((self.total,), np.float32)
+
+    def pack(self, arrays) -> np.ndarray:
+        return np.concatenate([np.asarray(a, dtype=np.complex128).ravel() for a in arrays])
+
+    def unpack(self, flat: np.ndarray):
+        return [flat[o:o + (1 << int(m))] for o, m in zip(self.offsets, self.log2n)]
+
+    def convolve(self, x: DeviceComplex, k: DeviceComplex, y: DeviceComplex):
+        _check(lib.pacg_batch_convolve(self.ptr, x.re.ptr, x.im.ptr, k.re.ptr, k.im.ptr, y.re.ptr, y.im.ptr),
+               "pacg_batch_convolve")
+
+    def autotune(self, x: DeviceComplex, k: DeviceComplex, y: DeviceComplex, reps: int = 3) -> int:
+        rc = lib.pacg_batch_autotune(self.ptr, x.re.ptr, x.im.ptr, k.re.ptr, k.im.ptr, y.re.ptr, y.im.ptr, int(reps))
+        if rc < 0:
+            raise PacgError("pacg_batch_autotune failed: %s" % last_error())
+        return rc
+
+    def set_variant(self, log2n: int, v: int):
+        _check(lib.pacg_batch_set_variant(self.ptr, int(log2n), int(v)), "pacg_batch_set_variant")
+
+    def variant(self, log2n: int) -> int:
+        return int(lib.pacg_batch_get_variant(self.ptr, int(log2n)))
+
+    @staticmethod
+    def num_variants(log2n: int) -> int:
+        return int(lib.pacg_batch_num_variants(int(log2n)))
+
+    @staticmethod
+    def schedule(log2n: int, v: int):
+        buf = (C.c_int * 32)()
+        ns = lib.pacg_batch_schedule(int(log2n), int(v), buf, 32)
+        if ns < 0:
+            raise PacgError(last_error())
+        return [buf[i] for i in range(ns)]
+
+    def capture(self, x: DeviceComplex, k: DeviceComplex, y: DeviceComplex):
+        _check(lib.pacg_batch_capture(self.ptr, x.re.ptr, x.im.ptr, k.re.ptr, k.im.ptr, y.re.ptr, y.im.ptr),
+               "pacg_batch_capture")
+
+    def replay(self):
+        _check(lib.pacg_batch_replay(self.ptr), "pacg_batch_replay")
+
+    def sync(self):
+        _check(lib.pacg_batch_sync(self.ptr), "pacg_batch_sync")
+
+    def close(self):
+        if getattr(self, "ptr", None):
+            lib.pacg_batch_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Timer:

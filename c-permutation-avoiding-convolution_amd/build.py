@@ -32,8 +32,8 @@ HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall",
              "-I", CSRC, "-I", INCLUDE]
 C_FLAGS = ["-O2", "-std=c11", "-fPIC", "-Wall", "-Wextra", "-Wno-unused-parameter",
            "-I", CSRC, "-I", INCLUDE]
-HOST_C = ["pacg_plan.c", "pacg_host.c", "pacg_dropin.c"]
-HIP_SRC = ["pacg_aux.hip"]
+HOST_C = ["pacg_plan.c", "pacg_host.c", "pacg_dropin.c", "pacg_batch.c"]
+HIP_SRC = ["pacg_aux.hip", "pacg_batch.hip"]
 
 
 def _newer(src: str, obj: str, deps) -> bool:
@@ -59,7 +59,7 @@ def build(jobs: int | None = None, incremental: bool = False, verbose: bool = Tr
     finally:
         sys.path.pop(0)
     gen_variants.main(GEN)
-    hdrs = [os.path.join(CSRC, h) for h in ("pacg_pass.hpp", "pacg_internal.h", "pacg_host.h")]
+    hdrs = [os.path.join(CSRC, h) for h in ("pacg_pass.hpp", "pacg_internal.h", "pacg_host.h", "pacg_batch.h")]
     hdrs += [os.path.join(INCLUDE, h) for h in ("pacg.h", "pacg_fft.h")]
     tasks = []
     objs = []

@@ -47,7 +47,7 @@ __global__ void permute_kernel(const T* __restrict__ ir, const T* __restrict__ i
   for (int a = d.nd - 1; a >= 0; --a) {
     const uint32_t i = (uint32_t)(rem % d.dims[a]);
     rem /= d.dims[a];
-    acc += (uint64_t)rev_axis(i, d.nst[a], d.radix_log2[a], d.dims[a]) * mul;
+    acc += (uint64_t)(d.nst[a] ? rev_axis(i, d.nst[a], d.radix_log2[a], d.dims[a]) : i) * mul;
     mul *= d.dims[a];
   }
   if (scatter) {

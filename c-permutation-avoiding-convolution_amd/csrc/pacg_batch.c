@@ -1,0 +1,322 @@
+/*
+ * Batched uneven-length convolutions, host side: packing, work items, the
+ * per-length schedule autotuner and HIP-graph capture of the single
+ * persistent launch (kernel: pacg_batch.hip).
+ *
+ * Pairs are packed back to back in input order (pair i at offset
+ * sum_{j<i} 2^m_j of each of x.re, x.im, k.re, k.im, y.re, y.im).  The
+ * kernel sees them sorted longest first and cut into items of up to
+ * (1024 threads * points per thread) / n pairs of one length n.
+ */
+#define _GNU_SOURCE
+#include <stdlib.h>
+#include <string.h>
+
+#include "pacg.h"
+#include "pacg_batch.h"
+#include "pacg_host.h"
+#include "pacg_internal.h"
+
+#define LMIN 8
+#define LMAX 14
+#define BATCH_THREADS 1024
+#define SLAB_POINTS (1 << 14)
+
+struct pacg_batch {
+  int count;
+  int* log2n;           /* input order */
+  long long* offset;    /* input order */
+  long long total;
+  int* order;           /* sorted position -> input index */
+  int variant[LMAX + 1];
+  int blocks;           /* workgroups of the persistent launch */
+  void* stream;
+  int own_stream;
+  void* graph;
+  /* device */
+  uint32_t* d_conv_off;
+  pacg_batch_item_t* d_items;
+  int n_items;
+  int cap_items;
+  void* d_tw;
+  void* d_slab;
+};
+
+static const pacg_batch_variant_t* variant_row(int m, int v) {
+  for (int i = 0; i < pacg_num_batch_variants; ++i)
+    if (pacg_batch_variants[i].log2n == m && pacg_batch_variants[i].variant == v) return &pacg_batch_variants[i];
+  return NULL;
+}
+
+int pacg_batch_schedule(int log2n, int variant, int* radix, int maxn) {
+  const pacg_batch_variant_t* r = variant_row(log2n, variant);
+  if (!r) return -pacg_fail(PACG_ERR_ARG, "no batched variant %d for length 2^%d", variant, log2n);
+  for (int q = 0; q < r->nstages && q < maxn; ++q) radix[q] = r->radix[q];
+  return r->nstages;
+}
+
+int pacg_batch_num_variants(int log2n) {
+  int c = 0;
+  for (int i = 0; i < pacg_num_batch_variants; ++i) c += pacg_batch_variants[i].log2n == log2n;
+  return c;
+}
+
+static int per_item(int m, int v) {
+  const pacg_batch_variant_t* r = variant_row(m, v);
+  if (!r) return 1;
+  int g = BATCH_THREADS / r->tpl;
+  return g < 1 ? 1 : g;
+}
+
+/* items for the lengths whose mask bit is set; uploads them */
+static int build_items(pacg_batch* b, unsigned mask) {
+  int n = 0;
+  pacg_batch_item_t* items = (pacg_batch_item_t*)malloc((size_t)(b->count + 1) * sizeof(pacg_batch_item_t));
+  if (!items) return pacg_fail(PACG_ERR_NOMEM, "out of host memory");
+  int pos = 0;
+  while (pos < b->count) {
+    int m = b->log2n[b->order[pos]];
+    int end = pos;
+    while (end < b->count && b->log2n[b->order[end]] == m) ++end;
+    if (mask & (1u << m)) {
+      int g = per_item(m, b->variant[m]);
+      for (int f = pos; f < end; f += g) {
+        items[n].first = (uint32_t)f;
+        items[n].count = (uint16_t)((end - f) < g ? (end - f) : g);
+        items[n].log2n = (uint8_t)m;
+        items[n].variant = (uint8_t)b->variant[m];
+        ++n;
+      }
+    }
+    pos = end;
+  }
+  int rc = PACG_OK;
+  if (n > 0 && pacg_memcpy_h2d(b->d_items, items, (size_t)n * sizeof(pacg_batch_item_t), b->stream))
+    rc = pacg_fail(PACG_ERR_HIP, "item upload failed");
+  if (!rc && pacg_stream_sync(b->stream)) rc = pacg_fail(PACG_ERR_HIP, "item upload failed");
+  free(items);
+  b->n_items = n;
+  return rc;
+}
+
+static int cmp_order_ctx_len;
+static const int* cmp_len;
+static int cmp_desc(const void* a, const void* b) {
+  int ia = *(const int*)a, ib = *(const int*)b;
+  if (cmp_len[ia] != cmp_len[ib]) return cmp_len[ib] - cmp_len[ia];
+  return ia - ib;
+}
+
+int pacg_batch_destroy(pacg_batch* b) {
+  if (!b) return PACG_OK;
+  if (b->stream) pacg_stream_sync(b->stream);
+  if (b->graph) pacg_graph_destroy(b->graph);
+  pacg_dev_free(b->d_conv_off);
+  pacg_dev_free(b->d_items);
+  pacg_dev_free(b->d_tw);
+  pacg_dev_free(b->d_slab);
+  if (b->own_stream && b->stream) pacg_stream_destroy(b->stream);
+  free(b->log2n);
+  free(b->offset);
+  free(b->order);
+  free(b);
+  return PACG_OK;
+}
+
+int pacg_batch_create(pacg_batch** out, int count, const int* log2_len) {
+  if (!out || count < 1 || !log2_len) return pacg_fail(PACG_ERR_ARG, "pacg_batch_create: bad argument");
+  *out = NULL;
+  pacg_batch* b = (pacg_batch*)calloc(1, sizeof *b);
+  if (!b) return pacg_fail(PACG_ERR_NOMEM, "out of host memory");
+  b->count = count;
+  b->log2n = (int*)malloc((size_t)count * sizeof(int));
+  b->offset = (long long*)malloc((size_t)count * sizeof(long long));
+  b->order = (int*)malloc((size_t)count * sizeof(int));
+  if (!b->log2n || !b->offset || !b->order) {
+    pacg_batch_destroy(b);
+    return pacg_fail(PACG_ERR_NOMEM, "out of host memory");
+  }
+  long long off = 0;
+  for (int i = 0; i < count; ++i) {
+    int m = log2_len[i];
+    if (m < LMIN || m > LMAX) {
+      pacg_batch_destroy(b);
+      return pacg_fail(PACG_ERR_SIZE, "pair %d: length 2^%d outside 2^%d..2^%d", i, m, LMIN, LMAX);
+    }
+    b->log2n[i] = m;
+    b->offset[i] = off;
+    b->order[i] = i;
+    off += 1LL << m;
+  }
+  b->total = off;
+  if (off >= (1LL << 32)) {
+    pacg_batch_destroy(b);
+    return pacg_fail(PACG_ERR_SIZE, "batch of %lld points exceeds 2^32", off);
+  }
+  cmp_len = b->log2n;
+  (void)cmp_order_ctx_len;
+  qsort(b->order, (size_t)count, sizeof(int), cmp_desc);
+  for (int m = 0; m <= LMAX; ++m) b->variant[m] = 0;
+  int rc = PACG_OK, cus = 0;
+  if (pacg_stream_create(&b->stream)) {
+    rc = pacg_fail(PACG_ERR_HIP, "stream creation failed");
+    goto fail;
+  }
+  b->own_stream = 1;
+  if (pacg_cu_count(&cus) || cus < 1) cus = 256;
+  b->blocks = cus;
+  b->cap_items = count + 1;
+  if (pacg_dev_malloc((void**)&b->d_conv_off, (size_t)count * sizeof(uint32_t)) ||
+      pacg_dev_malloc((void**)&b->d_items, (size_t)b->cap_items * sizeof(pacg_batch_item_t)) ||
+      pacg_dev_malloc(&b->d_tw, (size_t)SLAB_POINTS * 2 * sizeof(float)) ||
+      pacg_dev_malloc(&b->d_slab, (size_t)cus * SLAB_POINTS * 2 * sizeof(float))) {
+    rc = pacg_fail(PACG_ERR_NOMEM, "device allocation failed");
+    goto fail;
+  }
+  {
+    uint32_t* co = (uint32_t*)malloc((size_t)count * sizeof(uint32_t));
+    if (!co) {
+      rc = pacg_fail(PACG_ERR_NOMEM, "out of host memory");
+      goto fail;
+    }
+    for (int p = 0; p < count; ++p) co[p] = (uint32_t)b->offset[b->order[p]];
+    int e = pacg_memcpy_h2d(b->d_conv_off, co, (size_t)count * sizeof(uint32_t), b->stream);
+    e |= pacg_stream_sync(b->stream);
+    free(co);
+    if (e) {
+      rc = pacg_fail(PACG_ERR_HIP, "offset upload failed");
+      goto fail;
+    }
+  }
+  /* w_{2^14} in FP64 on the device, rounded to FP32 (twiddle_factors.c:17-131 values) */
+  if (pacg_gen_twiddles(b->d_tw, PACG_F32, SLAB_POINTS, SLAB_POINTS, 1, b->stream)) {
+    rc = pacg_fail(PACG_ERR_HIP, "twiddle generation failed");
+    goto fail;
+  }
+  if ((rc = build_items(b, ~0u))) goto fail;
+  *out = b;
+  pacg_clear_error();
+  return PACG_OK;
+fail:
+  pacg_batch_destroy(b);
+  return rc;
+}
+
+long long pacg_batch_total(const pacg_batch* b) { return b ? b->total : 0; }
+long long pacg_batch_offset(const pacg_batch* b, int i) { return (b && i >= 0 && i < b->count) ? b->offset[i] : -1; }
+void* pacg_batch_stream(const pacg_batch* b) { return b ? b->stream : NULL; }
+int pacg_batch_items(const pacg_batch* b) { return b ? b->n_items : 0; }
+
+int pacg_batch_get_variant(const pacg_batch* b, int log2n) {
+  if (!b || log2n < LMIN || log2n > LMAX) return -1;
+  return b->variant[log2n];
+}
+
+int pacg_batch_set_variant(pacg_batch* b, int log2n, int variant) {
+  if (!b || log2n < LMIN || log2n > LMAX || !variant_row(log2n, variant))
+    return pacg_fail(PACG_ERR_ARG, "no batched variant %d for length 2^%d", variant, log2n);
+  b->variant[log2n] = variant;
+  if (b->graph) {
+    pacg_graph_destroy(b->graph);
+    b->graph = NULL;
+  }
+  return build_items(b, ~0u);
+}
+
+static int launch(pacg_batch* b, const float* xr, const float* xi, const float* kr, const float* ki, float* yr,
+                  float* yi) {
+  pacg_batch_args_t a;
+  memset(&a, 0, sizeof a);
+  a.xr = xr;
+  a.xi = xi;
+  a.kr = kr;
+  a.ki = ki;
+  a.yr = yr;
+  a.yi = yi;
+  a.conv_off = b->d_conv_off;
+  a.items = b->d_items;
+  a.n_items = (uint32_t)b->n_items;
+  a.tw = b->d_tw;
+  a.spec_slab = b->d_slab;
+  int blocks = b->n_items < b->blocks ? b->n_items : b->blocks;
+  int rc = pacg_batch_launch(&a, blocks, b->stream);
+  if (rc) return pacg_fail(PACG_ERR_HIP, "batched launch failed: %s", pacg_hip_error_string(rc));
+  return PACG_OK;
+}
+
+int pacg_batch_convolve(pacg_batch* b, const float* xr, const float* xi, const float* kr, const float* ki, float* yr,
+                        float* yi) {
+  if (!b || !xr || !xi || !kr || !ki || !yr || !yi) return pacg_fail(PACG_ERR_ARG, "pacg_batch_convolve: NULL");
+  return launch(b, xr, xi, kr, ki, yr, yi);
+}
+
+int pacg_batch_autotune(pacg_batch* b, const float* xr, const float* xi, const float* kr, const float* ki, float* yr,
+                        float* yi, int reps) {
+  if (!b || !xr || !xi || !kr || !ki || !yr || !yi || reps < 1) return -pacg_fail(PACG_ERR_ARG, "bad argument");
+  void *e0 = NULL, *e1 = NULL;
+  if (pacg_event_create(&e0) || pacg_event_create(&e1)) return -pacg_fail(PACG_ERR_HIP, "event creation failed");
+  int changed = 0, rc = PACG_OK;
+  for (int m = LMIN; m <= LMAX && !rc; ++m) {
+    int present = 0;
+    for (int i = 0; i < b->count; ++i) present |= b->log2n[i] == m;
+    if (!present) continue;
+    int best = b->variant[m];
+    float best_ms = 1e30f;
+    for (int v = 0; v < pacg_batch_num_variants(m) && !rc; ++v) {
+      b->variant[m] = v;
+      if ((rc = build_items(b, 1u << m))) break;
+      if ((rc = launch(b, xr, xi, kr, ki, yr, yi))) break;
+      pacg_event_record(e0, b->stream);
+      for (int r = 0; r < reps && !rc; ++r) rc = launch(b, xr, xi, kr, ki, yr, yi);
+      pacg_event_record(e1, b->stream);
+      if (pacg_stream_sync(b->stream)) rc = pacg_fail(PACG_ERR_HIP, "autotune timing failed");
+      float ms = 0.0f;
+      pacg_event_elapsed_ms(&ms, e0, e1);
+      if (!rc && ms < best_ms) {
+        best_ms = ms;
+        best = v;
+      }
+    }
+    if (best != 0) ++changed;
+    b->variant[m] = best;
+  }
+  pacg_event_destroy(e0);
+  pacg_event_destroy(e1);
+  int rc2 = build_items(b, ~0u);
+  if (b->graph) {
+    pacg_graph_destroy(b->graph);
+    b->graph = NULL;
+  }
+  if (rc) return -rc;
+  if (rc2) return -rc2;
+  return changed;
+}
+
+int pacg_batch_capture(pacg_batch* b, const float* xr, const float* xi, const float* kr, const float* ki, float* yr,
+                       float* yi) {
+  if (!b) return pacg_fail(PACG_ERR_ARG, "NULL batch");
+  if (b->graph) {
+    pacg_graph_destroy(b->graph);
+    b->graph = NULL;
+  }
+  if (pacg_graph_begin(b->stream)) return pacg_fail(PACG_ERR_HIP, "begin capture failed");
+  int rc = launch(b, xr, xi, kr, ki, yr, yi);
+  void* exec = NULL;
+  int rc2 = pacg_graph_end(b->stream, &exec);
+  if (rc) return rc;
+  if (rc2) return pacg_fail(PACG_ERR_HIP, "end capture failed: %s", pacg_hip_error_string(rc2));
+  b->graph = exec;
+  return PACG_OK;
+}
+
+int pacg_batch_replay(pacg_batch* b) {
+  if (!b || !b->graph) return pacg_fail(PACG_ERR_STATE, "no captured batch graph");
+  if (pacg_graph_launch(b->graph, b->stream)) return pacg_fail(PACG_ERR_HIP, "graph launch failed");
+  return PACG_OK;
+}
+
+int pacg_batch_sync(pacg_batch* b) {
+  if (!b) return pacg_fail(PACG_ERR_ARG, "NULL batch");
+  if (pacg_stream_sync(b->stream)) return pacg_fail(PACG_ERR_HIP, "stream synchronize failed");
+  return PACG_OK;
+}

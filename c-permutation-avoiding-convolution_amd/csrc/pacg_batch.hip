@@ -28,28 +28,38 @@ constexpr int BLD = BPTS + BPTS / 32; // padded LDS length per array
 
 template <class SH>
 __device__ __forceinline__ void batch_item(const pacg_batch_args_t& a, const pacg_batch_item_t it, float* lre,
-                                           float* lim) {
+                                           float* lim, const float2* twh, const float2* twl_) {
   constexpr int P = SH::P, E = SH::E, TPL = SH::TPL, NP = SH::NP;
   static_assert(BT % TPL == 0, "threads per convolution");
   constexpr int G = BT / TPL;  // convolution slots per item
   constexpr int LDP = P + P / 32;
-  const int tid = threadIdx.x;
+  // Recompute every per-thread index inside the persistent loop: hoisted out
+  // of it, the positions and LDS addresses of all phases would stay live.
+  int tid = threadIdx.x;
+  asm volatile("" : "+v"(tid));
   const int g = tid / TPL;
   const int tl = tid % TPL;
   const bool valid = g < (int)it.count;
   const uint32_t base = valid ? a.conv_off[it.first + g] : 0u;
-  const float2* __restrict__ tw = reinterpret_cast<const float2*>(a.tw);
-  constexpr int TW_SHIFT = 14 - SH::LGP;  // w_P^idx = w_{2^14}^{idx << TW_SHIFT}
+  // w_P^idx = w_{2^14}^e, e = idx << TW_SHIFT = hi[e >> 7] * lo[e & 127] (two LDS tables)
+  constexpr int TW_SHIFT = 14 - SH::LGP;
   auto twl = [&](int idx, float& wr, float& wi) PACG_INL {
-    const float2 w = tw[idx << TW_SHIFT];
-    wr = w.x;
-    wi = w.y;
+    const int e = idx << TW_SHIFT;
+    const float2 h = twh[e >> 7];
+    const float2 l = twl_[e & 127];
+    wr = h.x * l.x - h.y * l.y;
+    wi = h.x * l.y + h.y * l.x;
   };
   auto li = [&](int s) PACG_INL { return g * LDP + s + (s >> 5); };
 
+  // Spectrum A^T k: in registers, except at 16 points per thread where it is
+  // parked in this workgroup's slab of a.spec_slab (each thread re-reads only
+  // what it wrote, so no barrier is involved; the slab stays in L2).
+  constexpr bool SPEC_REG = E <= 8;
   float vr[E], vi[E];   // working points
-  float pr[E], pi[E];   // x prefetched while k is transformed
-  float sr[E], si[E];   // spectrum A^T k (phase-0 layout)
+  float pr[(E <= 8 && P <= 4096) ? E : 1], pi[(E <= 8 && P <= 4096) ? E : 1];  // x prefetch
+  float sr[SPEC_REG ? E : 1], si[SPEC_REG ? E : 1];
+  float2* slab = reinterpret_cast<float2*>(a.spec_slab) + (size_t)blockIdx.x * BPTS;
 
   auto load_top = [&](const float* __restrict__ br, const float* __restrict__ bi, float* dr, float* di) PACG_INL {
     constexpr int PH = NP - 1;
@@ -99,21 +109,41 @@ __device__ __forceinline__ void batch_item(const pacg_batch_args_t& a, const pac
     });
   };
 
+  // Up to 2^12 points with 8 per thread, x is loaded while k is transformed;
+  // beyond that the registers hold the spectrum and x is loaded afterwards.
+  constexpr bool PF = E <= 8 && P <= 4096;
   load_top(a.kr, a.ki, vr, vi);
-  load_top(a.xr, a.xi, pr, pi);
+  if constexpr (PF) load_top(a.xr, a.xi, pr, pi);
   dif();  // A^T k
+  if constexpr (SPEC_REG) {
 #pragma unroll
-  for (int e = 0; e < E; ++e) {
-    sr[e] = vr[e];
-    si[e] = vi[e];
-    vr[e] = pr[e];
-    vi[e] = pi[e];
+    for (int e = 0; e < E; ++e) {
+      sr[e] = vr[e];
+      si[e] = vi[e];
+      if constexpr (PF) {
+        vr[e] = pr[e];
+        vi[e] = pi[e];
+      }
+    }
+  } else {
+#pragma unroll
+    for (int e = 0; e < E; ++e) slab[e * BT + tid] = make_float2(vr[e], vi[e]);
   }
+  if constexpr (!PF) load_top(a.xr, a.xi, vr, vi);
   dif();  // A^T x
   const float scale = 1.0f / (float)P;
 #pragma unroll
   for (int e = 0; e < E; ++e) {
-    cmul<float>(vr[e], vi[e], sr[e], si[e]);
+    float hr, hi;
+    if constexpr (SPEC_REG) {
+      hr = sr[e];
+      hi = si[e];
+    } else {
+      const float2 h = slab[e * BT + tid];
+      hr = h.x;
+      hi = h.y;
+    }
+    cmul<float>(vr[e], vi[e], hr, hi);
     vr[e] *= scale;
     vi[e] *= scale;
   }
@@ -126,14 +156,16 @@ __device__ __forceinline__ void batch_item(const pacg_batch_args_t& a, const pac
   {
     constexpr int PH = NP - 1;
     constexpr int R = 1 << SH::lgR(PH);
+    uint32_t sb = base;
+    asm volatile("" : "+v"(sb));  // keep the store addresses from being formed early
     sfor<E / R>([&](auto c_) PACG_INL {
       constexpr int c = decltype(c_)::value;
       sfor<R>([&](auto r_) PACG_INL {
         constexpr int r = decltype(r_)::value;
-        const int s = pos<SH, PH>(tl, c, r);
+        const uint32_t s = sb + (uint32_t)pos<SH, PH>(tl, c, r);
         if (valid) {
-          a.yr[base + s] = vr[c * R + r];
-          a.yi[base + s] = vi[c * R + r];
+          a.yr[s] = vr[c * R + r];
+          a.yi[s] = vi[c * R + r];
         }
       });
     });
@@ -141,35 +173,163 @@ __device__ __forceinline__ void batch_item(const pacg_batch_args_t& a, const pac
   __syncthreads();  // LDS is reused by the next item
 }
 
+// Lines longer than BT * 8 points: each thread runs VT = TPL / BT virtual
+// threads of 8 points in turn, the line stays in LDS between phases (the same
+// LDS traffic as the exchange model, half the live registers) and the
+// spectrum is parked in the workgroup's slab.  One convolution per item.
+template <class SH>
+__device__ __forceinline__ void batch_item_vt(const pacg_batch_args_t& a, const pacg_batch_item_t it, float* lre,
+                                              float* lim, const float2* twh, const float2* twl_) {
+  constexpr int P = SH::P, E = SH::E, NP = SH::NP;
+  constexpr int VT = SH::TPL / BT;
+  static_assert(VT * BT == SH::TPL && E <= 8, "virtual threads");
+  int tid = threadIdx.x;
+  asm volatile("" : "+v"(tid));
+  const bool valid = it.count > 0;
+  const uint32_t base = valid ? a.conv_off[it.first] : 0u;
+  constexpr int TW_SHIFT = 14 - SH::LGP;
+  auto twl = [&](int idx, float& wr, float& wi) PACG_INL {
+    const int e = idx << TW_SHIFT;
+    const float2 h = twh[e >> 7];
+    const float2 l = twl_[e & 127];
+    wr = h.x * l.x - h.y * l.y;
+    wi = h.x * l.y + h.y * l.x;
+  };
+  auto li = [&](int s) PACG_INL { return s + (s >> 5); };
+  float2* slab = reinterpret_cast<float2*>(a.spec_slab) + (size_t)blockIdx.x * BPTS;
+  const float scale = 1.0f / (float)P;
+
+  // one phase for one virtual thread: read (global or LDS), compute, write back
+  auto run = [&](auto PH_, auto MODE_, auto FROMG_, auto TAIL_, int vtl, const float* __restrict__ gr,
+                 const float* __restrict__ gi) PACG_INL {
+    constexpr int PH = decltype(PH_)::value;
+    constexpr int MODE = decltype(MODE_)::value;
+    constexpr bool from_global = decltype(FROMG_)::value;
+    constexpr int tail = decltype(TAIL_)::value;
+    constexpr int R = 1 << SH::lgR(PH);
+    float vr[E], vi[E];
+    sfor<E / R>([&](auto c_) PACG_INL {
+      constexpr int c = decltype(c_)::value;
+      sfor<R>([&](auto r_) PACG_INL {
+        constexpr int r = decltype(r_)::value;
+        const int sp = pos<SH, PH>(vtl, c, r);
+        if constexpr (from_global) {
+          vr[c * R + r] = valid ? gr[base + sp] : 0.f;
+          vi[c * R + r] = valid ? gi[base + sp] : 0.f;
+        } else {
+          vr[c * R + r] = lre[li(sp)];
+          vi[c * R + r] = lim[li(sp)];
+        }
+      });
+    });
+    if constexpr (MODE == M_DIF) {
+      phase_stages<float, SH, PH, M_DIF>(vr, vi);
+      phase_twiddle_f<float, SH, PH, false>(vr, vi, vtl, twl);
+    } else {
+      phase_twiddle_f<float, SH, PH, true>(vr, vi, vtl, twl);
+      phase_stages<float, SH, PH, M_DITC>(vr, vi);
+    }
+    // tail: 0 write LDS, 1 k spectrum -> slab, 2 x: product + conj(A) phase 0 -> LDS, 3 -> y
+    sfor<E / R>([&](auto c_) PACG_INL {
+      constexpr int c = decltype(c_)::value;
+      if constexpr (tail == 2) {
+        sfor<R>([&](auto r_) PACG_INL {
+          constexpr int r = decltype(r_)::value;
+          const float2 h = slab[pos<SH, PH>(vtl, c, r)];
+          cmul<float>(vr[c * R + r], vi[c * R + r], h.x, h.y);
+          vr[c * R + r] *= scale;
+          vi[c * R + r] *= scale;
+        });
+      }
+    });
+    if constexpr (PH == 0) {
+      if constexpr (tail == 2) phase_stages<float, SH, 0, M_DITC>(vr, vi);
+    }
+    sfor<E / R>([&](auto c_) PACG_INL {
+      constexpr int c = decltype(c_)::value;
+      sfor<R>([&](auto r_) PACG_INL {
+        constexpr int r = decltype(r_)::value;
+        const int sp = pos<SH, PH>(vtl, c, r);
+        if constexpr (tail == 1) {
+          slab[sp] = make_float2(vr[c * R + r], vi[c * R + r]);
+        } else if constexpr (tail == 3) {
+          if (valid) {
+            a.yr[base + sp] = vr[c * R + r];
+            a.yi[base + sp] = vi[c * R + r];
+          }
+        } else {
+          lre[li(sp)] = vr[c * R + r];
+          lim[li(sp)] = vi[c * R + r];
+        }
+      });
+    });
+  };
+
+  // A^T k (spectrum to the slab), then A^T x, product, conj(A).  The virtual
+  // threads run one after the other (no unrolling: one set of points live).
+  sfor<2>([&](auto pass_) PACG_INL {
+    constexpr int pass = decltype(pass_)::value;
+    const float* gr = pass == 0 ? a.kr : a.xr;
+    const float* gi = pass == 0 ? a.ki : a.xi;
+    sfor<NP>([&](auto i_) PACG_INL {
+      constexpr int PH = NP - 1 - decltype(i_)::value;
+      constexpr int tail = PH > 0 ? 0 : (pass == 0 ? 1 : 2);
+      __syncthreads();
+#pragma nounroll
+      for (int v = 0; v < VT; ++v)
+        run(std::integral_constant<int, PH>{}, std::integral_constant<int, M_DIF>{},
+            std::integral_constant<bool, PH == NP - 1>{}, std::integral_constant<int, tail>{}, tid + v * BT, gr, gi);
+    });
+  });
+  sfor<NP - 1>([&](auto i_) PACG_INL {
+    constexpr int PH = 1 + decltype(i_)::value;
+    constexpr int tail = PH == NP - 1 ? 3 : 0;
+    __syncthreads();
+#pragma nounroll
+    for (int v = 0; v < VT; ++v)
+      run(std::integral_constant<int, PH>{}, std::integral_constant<int, M_DITC>{}, std::integral_constant<bool, false>{},
+          std::integral_constant<int, tail>{}, tid + v * BT, a.xr, a.xi);
+  });
+  __syncthreads();  // LDS is reused by the next item
+}
+
 // Variant table: (log2 n, variant) -> shape.  Stage 0 first; phases listed
 // by their top stage.  Kept in step with pacg_batch_variants[] below.
-#define PACG_BATCH_VARIANTS(X)                                    \
-  X(8, 0, 4, 0xAAu, 2, 0x31u)         /* 4 4 | 4 4      */ \
-  X(8, 1, 3, 0x2Fu, 3, 0x210u)        /* 8 | 8 | 4      */ \
-  X(9, 0, 3, 0x3Fu, 3, 0x210u)        /* 8 | 8 | 8      */ \
-  X(9, 1, 5, 0x2A9u, 3, 0x431u)       /* 2 4 | 4 4 | 4  */ \
-  X(10, 0, 5, 0x2AAu, 3, 0x431u)      /* 4 4 | 4 4 | 4  */ \
-  X(10, 1, 4, 0xFDu, 3, 0x321u)       /* 2 8 | 8 | 8    */ \
-  X(10, 2, 4, 0xAFu, 3, 0x310u)       /* 8 | 8 | 4 4    */ \
-  X(11, 0, 6, 0x6AAu, 3, 0x531u)      /* 4 4 | 4 4 | 4 2 */ \
-  X(11, 1, 4, 0xBFu, 4, 0x3210u)      /* 8 | 8 | 8 | 4  */ \
-  X(12, 0, 6, 0xAAAu, 3, 0x531u)      /* 4 4 | 4 4 | 4 4 */ \
-  X(12, 1, 4, 0xFFu, 4, 0x3210u)      /* 8 | 8 | 8 | 8  */ \
-  X(13, 0, 7, 0x1AAAu, 4, 0x6531u)    /* 4 4 |4 4 |4 4 | 2 */ \
-  X(13, 1, 5, 0x1FFu, 4, 0x4210u)     /* 8 | 8 | 8 | 8 2 */ \
-  X(14, 0, 7, 0x2AAAu, 4, 0x6531u)    /* 4 4 |4 4 |4 4 | 4 */ \
-  X(14, 1, 5, 0x3FDu, 4, 0x4321u)     /* 2 8 | 8 | 8 | 8 */
+#define PACG_BATCH_VARIANTS(X)                                          \
+  X(8, 0, 3, 0x2Fu, 3, 0x210u)        /* 8 | 8 | 4             */     \
+  X(8, 1, 4, 0xAAu, 4, 0x3210u)       /* 4 | 4 | 4 | 4         */     \
+  X(9, 0, 3, 0x3Fu, 3, 0x210u)        /* 8 | 8 | 8             */     \
+  X(9, 1, 4, 0xABu, 4, 0x3210u)       /* 8 | 4 | 4 | 4         */     \
+  X(10, 0, 4, 0x7Fu, 4, 0x3210u)      /* 8 | 8 | 8 | 2         */     \
+  X(10, 1, 4, 0xAFu, 4, 0x3210u)      /* 8 | 8 | 4 | 4         */     \
+  X(11, 0, 4, 0xBFu, 4, 0x3210u)      /* 8 | 8 | 8 | 4         */     \
+  X(11, 1, 5, 0x2F9u, 4, 0x4321u)     /* 2 4 | 8 | 8 | 4       */     \
+  X(12, 0, 4, 0xFFu, 4, 0x3210u)      /* 8 | 8 | 8 | 8         */     \
+  X(13, 0, 5, 0x1FFu, 5, 0x43210u)    /* 8 | 8 | 8 | 8 | 2     */     \
+  X(14, 0, 5, 0x3FEu, 5, 0x43210u)    /* 4 | 8 | 8 | 8 | 8 (two 8-point virtual threads) */
 
 __global__ void __launch_bounds__(BT) batch_kernel(const pacg_batch_args_t a) {
   __shared__ float lre[BLD];
   __shared__ float lim[BLD];
+  __shared__ float2 twh[128], twl[128];
+  {
+    const float2* tw = reinterpret_cast<const float2*>(a.tw);
+    const int t = threadIdx.x;
+    if (t < 128) twh[t] = tw[t * 128];
+    else if (t < 256) twl[t - 128] = tw[t - 128];
+  }
+  __syncthreads();
   for (uint32_t i = blockIdx.x; i < a.n_items; i += gridDim.x) {
     const pacg_batch_item_t it = a.items[i];
     const int key = (int)it.log2n * 4 + (int)it.variant;
     switch (key) {
-#define PACG_CASE(M, V, NS, SC, NPH, PC)                \
-  case M * 4 + V:                                      \
-    batch_item<Shape<NS, SC, NPH, PC>>(a, it, lre, lim); \
+#define PACG_CASE(M, V, NS, SC, NPH, PC)                                    \
+  case M * 4 + V:                                                          \
+    static_assert(Shape<NS, SC, NPH, PC>::LGP == M, "variant length");    \
+    if constexpr (Shape<NS, SC, NPH, PC>::P >= BT * 8)                     \
+      batch_item_vt<Shape<NS, SC, NPH, PC>>(a, it, lre, lim, twh, twl);    \
+    else                                                                   \
+      batch_item<Shape<NS, SC, NPH, PC>>(a, it, lre, lim, twh, twl);       \
     break;
       PACG_BATCH_VARIANTS(PACG_CASE)
 #undef PACG_CASE

@@ -295,7 +295,9 @@ __device__ __forceinline__ void phase_stages(T* vr, T* vi) {
 }
 
 // Phase-external twiddle w_{R lambda}^{mu k(r)}; tw(idx, wr, wi) returns w_P^idx.
-template <typename T, class SH, int PH, bool CJ, typename TW>
+// SERIAL: finish one group before the next group's table reads are issued
+// (keeps register pressure down where many groups share a thread).
+template <typename T, class SH, int PH, bool CJ, typename TW, bool SERIAL = false>
 __device__ __forceinline__ void phase_twiddle_f(T* vr, T* vi, int tl, TW&& tw) {
   constexpr int lo = SH::plo(PH);
   if constexpr (lo > 0) {
@@ -318,6 +320,7 @@ __device__ __forceinline__ void phase_twiddle_f(T* vr, T* vi, int tl, TW&& tw) {
           cmul<T>(vr[c * R + r], vi[c * R + r], wr, wi);
         }
       });
+      if constexpr (SERIAL) __builtin_amdgcn_sched_barrier(0);
     });
   }
 }

@@ -103,6 +103,7 @@ struct pacg_plan {
   void* scr_im;
   void* graph;
   int precs[2]; /* which precisions have tables */
+  long long batch; /* length of a batch axis 0 (1 when none) */
 };
 
 static size_t esize(int prec) { return prec == PACG_F64 ? 8u : 4u; }
@@ -274,6 +275,10 @@ int pacg_plan_create(pacg_plan** out, const pacg_desc* d) {
   pl->nd = d->nd;
   pl->n = 1;
   for (int a = 0; a < d->nd; ++a) {
+    if (d->nstages[a] == -1 && a == 0 && d->dims[a] >= 1) {
+      pl->n *= d->dims[a];
+      continue;
+    }
     if (d->dims[a] < 1 || ilog2(d->dims[a]) < 0) {
       free(pl);
       return pacg_fail(PACG_ERR_SIZE, "axis %d length %d is not a power of two", a, d->dims[a]);
@@ -284,6 +289,7 @@ int pacg_plan_create(pacg_plan** out, const pacg_desc* d) {
     free(pl);
     return pacg_fail(PACG_ERR_SIZE, "total size %lld >= 2^31", pl->n);
   }
+  pl->batch = 1;
   pl->precs[PACG_F64] = 1;
   pl->precs[PACG_F32] = d->precision == PACG_PREC_F32;
   int rc = PACG_OK;
@@ -293,7 +299,15 @@ int pacg_plan_create(pacg_plan** out, const pacg_desc* d) {
     ax->inner = 1;
     for (int b = a + 1; b < d->nd; ++b) ax->inner *= d->dims[b];
     ax->outer = pl->n / ((long long)ax->N * ax->inner);
-    if (d->nstages[a] == 0) {
+    if (d->nstages[a] == -1) {
+      if (a != 0) {
+        rc = pacg_fail(PACG_ERR_ARG, "only axis 0 can be a batch axis");
+        goto fail;
+      }
+      ax->t = 0;
+      pl->batch = ax->N;
+      continue;
+    } else if (d->nstages[a] == 0) {
       ax->t = default_schedule(ax->N, ax->radix);
     } else {
       if (d->nstages[a] < 0 || d->nstages[a] > PACG_MAX_STAGES) {
@@ -378,7 +392,7 @@ int pacg_plan_rho(const pacg_plan* pl, int* rho) {
       const axis_t* ax = &pl->ax[a];
       long long i = rem % ax->N;
       rem /= ax->N;
-      acc += pacg_rev_index_mixed(i, ax->N, ax->radix, ax->t) * mul;
+      acc += (ax->t ? pacg_rev_index_mixed(i, ax->N, ax->radix, ax->t) : i) * mul;
       mul *= ax->N;
     }
     rho[idx] = (int)acc;
@@ -475,7 +489,7 @@ static int conv_launch(const pacg_plan* pl, int i, step_t* st, int* kind) {
  * launch i and after the last one. */
 static int run_conv_ev(pacg_plan* pl, int prec, const void* xr, const void* xi, void* yr, void* yi, const void* sr,
                        const void* si, int sprec, void** events) {
-  double scale = 1.0 / (double)pl->n;
+  double scale = (double)pl->batch / (double)pl->n;
   if (pl->nsteps == 0) {
     if (xr != yr) HIPCHK(pacg_convert(xr, xi, prec, yr, yi, prec, (uint64_t)pl->n, 1.0, pl->stream), "copy");
     HIPCHK(pacg_pointwise(yr, yi, sr, si, prec, sprec, (uint64_t)pl->n, scale, pl->stream), "pointwise");
@@ -532,7 +546,7 @@ int pacg_plan_autotune(pacg_plan* pl, const void* xr, const void* xi, void* yr, 
         ps->v[prec][kind][vsp] = v;
         const void* s_re = kind == PACG_K_FUSED ? pl->spec_re : NULL;
         const void* s_im = kind == PACG_K_FUSED ? pl->spec_im : NULL;
-        double scale = kind == PACG_K_FUSED ? 1.0 / (double)pl->n : 1.0;
+        double scale = kind == PACG_K_FUSED ? (double)pl->batch / (double)pl->n : 1.0;
         if ((rc = launch_step(pl, st, prec, kind, vsp, xr, xi, yr, yi, s_re, s_im, scale, 1.0))) break;
         if (pacg_event_record(e0, pl->stream)) rc = pacg_fail(PACG_ERR_HIP, "event record failed");
         for (int r = 0; r < reps && !rc; ++r)
@@ -622,7 +636,7 @@ static void perm_desc(const pacg_plan* pl, pacg_perm_desc_t* pd) {
 int pacg_execute(pacg_plan* pl, int op, const void* ir, const void* ii, void* orr, void* oi) {
   if (!pl || !ir || !ii || !orr || !oi) return pacg_fail(PACG_ERR_ARG, "pacg_execute: NULL argument");
   int prec = pidx(pl);
-  double inv_n = 1.0 / (double)pl->n;
+  double inv_n = (double)pl->batch / (double)pl->n;
   pacg_perm_desc_t pd;
   int rc;
   switch (op) {

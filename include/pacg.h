@@ -55,7 +55,9 @@ enum {
 /* Plan description.  Axis a has length dims[a] = prod radix[a][0..nstages[a]-1];
  * nstages[a] = 0 asks for the default schedule (radix-4 stages with one
  * radix-2 stage on top when log2 dims[a] is odd).  Stage 0 has unit stride;
- * its radix is the innermost digit of the digit reversal. */
+ * its radix is the innermost digit of the digit reversal.  nstages[0] = -1
+ * makes axis 0 a batch axis: any length, not transformed (independent
+ * convolutions of the rows, each scaled by 1 / its own size). */
 typedef struct {
   int nd;                                 /* 1, 2 or 3 */
   int dims[3];                            /* each a power of two, total < 2^31 */
@@ -114,6 +116,34 @@ int pacg_plan_autotune(pacg_plan* plan, const void* x_re, const void* x_im, void
  * the number of launches per convolution, or minus an error code. */
 int pacg_profile_convolve(pacg_plan* plan, const void* x_re, const void* x_im, void* y_re, void* y_im, int reps,
                           float* launch_ms, const char** names, int max_launches);
+
+/* ---- batched convolutions of uneven lengths (one persistent launch) --------
+ * count pairs (x_i, k_i) of lengths 2^m_i, 8 <= m_i <= 14, FP32 split complex,
+ * packed back to back in input order (pair i at pacg_batch_offset(b, i) in each
+ * of the six arrays).  y_i = (1/n_i) conj(A_i)[(A_i^T k_i) .* (A_i^T x_i)] with a
+ * per-length schedule chosen from compiled variants (pacg_batch_autotune). */
+typedef struct pacg_batch pacg_batch;
+int pacg_batch_create(pacg_batch** batch, int count, const int* log2_len);
+int pacg_batch_destroy(pacg_batch* batch);
+long long pacg_batch_total(const pacg_batch* batch);
+long long pacg_batch_offset(const pacg_batch* batch, int i);
+void* pacg_batch_stream(const pacg_batch* batch);
+int pacg_batch_items(const pacg_batch* batch);
+int pacg_batch_convolve(pacg_batch* batch, const float* x_re, const float* x_im, const float* k_re, const float* k_im,
+                        float* y_re, float* y_im);
+/* Times every compiled schedule of each length present on these buffers and
+ * keeps the fastest; returns how many lengths left variant 0, or -error. */
+int pacg_batch_autotune(pacg_batch* batch, const float* x_re, const float* x_im, const float* k_re,
+                        const float* k_im, float* y_re, float* y_im, int reps);
+int pacg_batch_set_variant(pacg_batch* batch, int log2n, int variant);
+int pacg_batch_get_variant(const pacg_batch* batch, int log2n);
+int pacg_batch_num_variants(int log2n);
+/* Stage radices (stage 0 first) of a variant; returns the stage count or -error. */
+int pacg_batch_schedule(int log2n, int variant, int* radix, int max_stages);
+int pacg_batch_capture(pacg_batch* batch, const float* x_re, const float* x_im, const float* k_re, const float* k_im,
+                       float* y_re, float* y_im);
+int pacg_batch_replay(pacg_batch* batch);
+int pacg_batch_sync(pacg_batch* batch);
 
 /* Free the device plans and buffers cached by the drop-in fft.h entry points. */
 void pacg_dropin_release(void);

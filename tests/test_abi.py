@@ -153,3 +153,14 @@ def test_product_package_does_not_touch_the_oracle():
             if f.endswith((".py", ".c", ".h", ".hip", ".hpp")):
                 text = open(os.path.join(dp, f)).read()
                 assert "oracle" not in text.lower() or f == "synth.py", f
+
+
+def test_batch_validation_needs_no_gpu(pacg):
+    with pytest.raises(pacg.PacgError, match="outside"):
+        pacg.Batch([8, 7, 12])
+    with pytest.raises(pacg.PacgError, match="outside"):
+        pacg.Batch([15])
+    for m in range(8, 15):
+        assert pacg.Batch.num_variants(m) >= 1
+        for v in range(pacg.Batch.num_variants(m)):
+            assert int(np.prod(pacg.Batch.schedule(m, v))) == 1 << m

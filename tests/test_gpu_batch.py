@@ -1,0 +1,104 @@
+"""Batched uneven-length convolutions (one persistent launch) and batch-axis
+plans, against the oracle pair by pair.  FP32 bar: rel-L2 <= 2e-6 per pair."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+TOL32 = 2e-6
+
+
+def rnd(seed, n):
+    g = np.random.default_rng(seed)
+    return g.uniform(-1, 1, n) + 1j * g.uniform(-1, 1, n)
+
+
+@pytest.fixture(scope="module")
+def small_batch(gpu):
+    ms = np.array([14, 8, 9, 13, 10, 11, 12, 8, 14, 12, 9, 10, 11, 13, 8, 8, 8, 10, 12, 14], dtype=np.int32)
+    xs = [rnd(100 + i, 1 << int(m)) for i, m in enumerate(ms)]
+    ks = [rnd(200 + i, 1 << int(m)) for i, m in enumerate(ms)]
+    b = gpu.Batch(ms)
+    dx = gpu.DeviceComplex.from_numpy(b.pack(xs), np.float32)
+    dk = gpu.DeviceComplex.from_numpy(b.pack(ks), np.float32)
+    dy = b.alloc()
+    return b, xs, ks, dx, dk, dy
+
+
+def check(b, xs, ks, dy, oracle, scheds):
+    ys = b.unpack(dy.numpy())
+    for i, (x, k, y) in enumerate(zip(xs, ks, ys)):
+        m = int(b.log2n[i])
+        ref, _ = oracle.pa_convolution(x, k, [scheds[m]])
+        assert rel_l2(y, ref) < TOL32, (i, m, rel_l2(y, ref))
+
+
+@pytest.mark.parametrize("log2n", list(range(8, 15)))
+def test_every_variant_of_every_length(gpu, oracle, small_batch, log2n):
+    b, xs, ks, dx, dk, dy = small_batch
+    for v in range(gpu.Batch.num_variants(log2n)):
+        b.set_variant(log2n, v)
+        dy.upload(np.zeros(b.total, complex))
+        b.convolve(dx, dk, dy)
+        b.sync()
+        scheds = {m: gpu.Batch.schedule(m, b.variant(m)) for m in range(8, 15)}
+        assert int(np.prod(scheds[log2n])) == 1 << log2n
+        check(b, xs, ks, dy, oracle, scheds)
+    b.set_variant(log2n, 0)
+
+
+def test_autotune_then_graph_replay(gpu, oracle, small_batch):
+    b, xs, ks, dx, dk, dy = small_batch
+    b.autotune(dx, dk, dy, 2)
+    scheds = {m: gpu.Batch.schedule(m, b.variant(m)) for m in range(8, 15)}
+    b.capture(dx, dk, dy)
+    for it in range(3):
+        dy.upload(np.zeros(b.total, complex))
+        b.replay()
+        b.sync()
+        check(b, xs, ks, dy, oracle, scheds)
+        b.convolve(dx, dk, dy)  # eager launch between replays
+        b.sync()
+
+
+def test_identity_and_shift_kernels(gpu):
+    ms = np.array([8, 11, 14], dtype=np.int32)
+    b = gpu.Batch(ms)
+    xs = [rnd(7 + i, 1 << int(m)) for i, m in enumerate(ms)]
+    ks = []
+    for i, m in enumerate(ms):
+        k = np.zeros(1 << int(m), complex)
+        k[i] = 1.0  # delta at i: y = roll(x, i)
+        ks.append(k)
+    dx = gpu.DeviceComplex.from_numpy(b.pack(xs), np.float32)
+    dk = gpu.DeviceComplex.from_numpy(b.pack(ks), np.float32)
+    dy = b.alloc()
+    b.convolve(dx, dk, dy)
+    b.sync()
+    for i, (x, y) in enumerate(zip(xs, b.unpack(dy.numpy()))):
+        assert np.abs(y - np.roll(x, i)).max() < 1e-5
+
+
+@pytest.mark.parametrize("prec,tol", [(0, 1e-12), (1, TOL32)])
+def test_batch_axis_plan_convolves_rows(gpu, oracle, prec, tol):
+    rows, n = 5, 512
+    sched = [8, 8, 8]
+    x = np.stack([rnd(300 + r, n) for r in range(rows)])
+    k = np.stack([rnd(400 + r, n) for r in range(rows)])
+    pl = gpu.Plan((rows, n), schedules=["batch", sched], precision=prec)
+    pl.set_kernel(k)
+    dt = np.float64 if prec == 0 else np.float32
+    dx = gpu.DeviceComplex.from_numpy(x, dt)
+    pl.convolve(dx)
+    y = dx.numpy()
+    for r in range(rows):
+        ref, _ = oracle.pa_convolution(x[r], k[r], [sched])
+        assert rel_l2(y[r], ref) < tol
+    for op in (gpu.OP_FFT, gpu.OP_IFFT):
+        dx = gpu.DeviceComplex.from_numpy(x, dt)
+        pl.execute(op, dx)
+        ref = np.fft.fft(x, axis=1) if op == gpu.OP_FFT else np.fft.ifft(x, axis=1)
+        assert rel_l2(dx.numpy(), ref) < tol
