@@ -113,6 +113,33 @@ def cpu_baseline(x, k, reps):
             "n": int(n)}
 
 
+def bench_mfma(pacg, synth, steps, warmup):
+    """4096 x 256 complex, radix 8-8-8-8 / 8-8-4, radix-8 stages on split-FP16 MFMA (seed 3),
+    beside the same plan on the VALU butterflies."""
+    x, k = synth.config_inputs(4)
+    n = x.size
+    scheds = [[8, 8, 8, 8], [8, 8, 4]]
+    ref = np.fft.ifft2(np.fft.fft2(x) * np.fft.fft2(k))
+    out = {"workload": "2D 4096x256 complex FP32, U(-1,1) seed 3, schedules 8-8-8-8 / 8-8-4"}
+    for name, mf in (("mfma", True), ("valu", False)):
+        pl = pacg.Plan(x.shape, schedules=scheds, precision=pacg.PREC_F32, spectrum_precision=pacg.PREC_F32,
+                       mfma=mf)
+        pl.set_kernel(k)
+        dx = pacg.DeviceComplex.from_numpy(x, np.float32)
+        dy = pl.alloc()
+        if not mf:
+            pl.autotune(dx, dy, 3)
+        ms = time_convs(pacg, pl, dx, dy, steps, warmup)
+        y = dy.numpy()
+        b_alg = 3 * 2 * n * 8 + n * 8
+        out[name] = {"ms_per_conv": ms, "conv_per_s": 1000.0 / ms, "launches": pl.launches_per_convolve(),
+                     "frac_of_B_alg": b_alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                     "rel_l2": float(np.linalg.norm(y - ref) / np.linalg.norm(ref)),
+                     "max_abs": float(np.abs(y - ref).max()), "plan": pl.describe()}
+        del pl, dx, dy
+    return out
+
+
 def bench_batched(pacg, synth, steps, warmup):
     """4096 FP32 pairs of lengths 2^U{8..14} (seed 4) in one persistent, graph-captured launch,
     against per-length plain FP64 launches (one batch-axis plan per length)."""
@@ -180,6 +207,9 @@ def main():
                     help="storage of the FP64-computed spectrum on the headline FP32 path")
     ap.add_argument("--no-tune", action="store_true", help="skip the tile autotuner")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched uneven-length workload")
+    ap.add_argument("--no-mfma", action="store_true", help="skip the 4096x256 MFMA workload")
+    ap.add_argument("--profile-mfma", action="store_true",
+                    help="only run the 4096x256 MFMA convolution `steps` times (for rocprofv3 counters)")
     ap.add_argument("--profile", action="store_true",
                     help="only run the FP32 convolution `steps` times after a copy kernel of 2 x 256 MiB "
                          "(counter calibration), for rocprofv3")
@@ -191,6 +221,19 @@ def main():
 
     pacg = load_pacg()
     import pacg.synth as synth
+
+    if a.profile_mfma:
+        x4, k4 = synth.config_inputs(4)
+        pl = pacg.Plan(x4.shape, schedules=[[8, 8, 8, 8], [8, 8, 4]], precision=pacg.PREC_F32,
+                       spectrum_precision=pacg.PREC_F32, mfma=True)
+        pl.set_kernel(k4)
+        dx = pacg.DeviceComplex.from_numpy(x4, np.float32)
+        dy = pl.alloc()
+        for _ in range(a.steps):
+            pl.convolve(dx, dy)
+        pl.sync()
+        print(json.dumps({"profile_mfma": True, "steps": a.steps, "plan": pl.describe()}))
+        return
 
     t0 = time.time()
     x, k = synth.config_inputs(3)
@@ -298,6 +341,9 @@ def main():
         "launches_fp32": [{"kernel": nm, "ms": t} for nm, t in prof32],
         "launches_fp64": [{"kernel": nm, "ms": t} for nm, t in prof64],
     }
+    if not a.no_mfma:
+        log("mfma workload ...")
+        out["mfma_4096x256"] = bench_mfma(pacg, synth, a.steps, a.warmup)
     if not a.no_batched:
         log("batched workload ...")
         out["batched"] = bench_batched(pacg, synth, a.steps, a.warmup)

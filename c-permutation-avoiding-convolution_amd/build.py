@@ -33,7 +33,7 @@ HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall",
 C_FLAGS = ["-O2", "-std=c11", "-fPIC", "-Wall", "-Wextra", "-Wno-unused-parameter",
            "-I", CSRC, "-I", INCLUDE]
 HOST_C = ["pacg_plan.c", "pacg_host.c", "pacg_dropin.c", "pacg_batch.c"]
-HIP_SRC = ["pacg_aux.hip", "pacg_batch.hip"]
+HIP_SRC = ["pacg_aux.hip", "pacg_batch.hip", "pacg_mfma.hip"]
 
 
 def _newer(src: str, obj: str, deps) -> bool:

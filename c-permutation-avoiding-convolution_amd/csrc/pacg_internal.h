@@ -69,12 +69,16 @@ typedef struct {
   int rank;              /* 0: the default tile for its key; > 0: autotuner alternatives */
   pacg_launch_fn launch;
   const char* name;
+  int engine;            /* 0: VALU butterflies, 1: radix-8 stages on MFMA (split FP16) */
 } pacg_variant_t;
 
 /* registry of every compiled pass kernel (generated: gen/pacg_variants_index.c) */
 extern const pacg_variant_t* const pacg_variant_tables[];
 extern const int pacg_variant_counts[];
 extern const int pacg_num_variant_tables;
+/* radix-8 MFMA pass kernels (pacg_mfma.hip) */
+extern pacg_variant_t pacg_mfma_variants[];
+extern const int pacg_num_mfma_variants;
 
 /* ---- auxiliary kernels (pacg_aux.hip) ---------------------------------- */
 /* K1: twiddle tables generated in FP64 on the device.

@@ -67,6 +67,7 @@ typedef struct {
   int q_lo, q_hi, P, L;
   uint32_t code;
   int layout;
+  int mfma; /* FP32 kernels of this pass come from the MFMA table */
   const pacg_variant_t* v[2][4][2]; /* [prec][kind][spec prec] */
   void* twp[2];
 } pass_t;
@@ -107,6 +108,16 @@ struct pacg_plan {
 };
 
 static size_t esize(int prec) { return prec == PACG_F64 ? 8u : 4u; }
+
+static const pacg_variant_t* find_mfma(int spec_prec, int kind, int layout, int ns, uint32_t code) {
+  for (int i = 0; i < pacg_num_mfma_variants; ++i) {
+    const pacg_variant_t* v = &pacg_mfma_variants[i];
+    if (v->kind == kind && v->layout == layout && v->nstages == ns && v->stage_code == code &&
+        (kind != PACG_K_FUSED || v->spec_prec == spec_prec))
+      return v;
+  }
+  return NULL;
+}
 
 static const pacg_variant_t* find_variant(int prec, int spec_prec, int kind, int layout, int ns, uint32_t code) {
   for (int t = 0; t < pacg_num_variant_tables; ++t) {
@@ -160,12 +171,22 @@ static int default_schedule(int N, int* radix) {
 
 /* Cut one axis' stages into passes, bottom up, taking the longest run with a
  * compiled kernel (P <= 512).  Single stages always have one. */
-static int split_axis(axis_t* ax, int need_f32) {
+static int split_axis(axis_t* ax, int need_f32, int mfma) {
   int q = 0;
   ax->npass = 0;
   while (q < ax->t) {
-    int best = 0;
-    for (int len = ax->t - q; len >= 1; --len) {
+    int best = 0, is_mfma = 0;
+    if (mfma && ax->radix[q] == 8) {
+      /* radix-8 stages go to the matrix cores, two per pass */
+      best = (q + 1 < ax->t && ax->radix[q + 1] == 8) ? 2 : 1;
+      is_mfma = 1;
+    }
+    for (int len = ax->t - q; len >= 1 && !best; --len) {
+      if (mfma) {
+        int has8 = 0;
+        for (int i = q; i < q + len; ++i) has8 |= ax->radix[i] == 8;
+        if (has8) continue;
+      }
       if (len > 12) continue;
       long long P = 1;
       for (int i = q; i < q + len; ++i) P *= ax->radix[i];
@@ -187,6 +208,7 @@ static int split_axis(axis_t* ax, int need_f32) {
     ps->L = 1;
     for (int i = 0; i < ps->q_lo; ++i) ps->L *= ax->radix[i];
     ps->code = run_code(ax->radix, ps->q_lo, ps->q_hi);
+    ps->mfma = is_mfma;
     q += best;
   }
   return PACG_OK;
@@ -204,8 +226,13 @@ static int bind_variants(pacg_plan* pl) {
         if (!pl->precs[pr]) continue;
         for (int k = 0; k < 4; ++k)
           for (int sp = 0; sp < 2; ++sp) {
-            const pacg_variant_t* v = find_variant(pr, sp, k, ps->layout, ns, ps->code);
-            if (!v && ps->layout == PACG_L_CONTIG) v = find_variant(pr, sp, k, PACG_L_STRIDED, ns, ps->code);
+            const pacg_variant_t* v = NULL;
+            if (ps->mfma && pr == PACG_F32) {
+              v = find_mfma(sp, k, ps->layout, ns, ps->code);
+            } else {
+              v = find_variant(pr, sp, k, ps->layout, ns, ps->code);
+              if (!v && ps->layout == PACG_L_CONTIG) v = find_variant(pr, sp, k, PACG_L_STRIDED, ns, ps->code);
+            }
             ps->v[pr][k][sp] = v;
           }
         if (!ps->v[pr][PACG_K_DIF][pr] || !ps->v[pr][PACG_K_DITC][pr] || !ps->v[pr][PACG_K_DIT][pr])
@@ -330,7 +357,7 @@ int pacg_plan_create(pacg_plan** out, const pacg_desc* d) {
       }
     }
     if (ax->N == 1) ax->t = 0;
-    if ((rc = split_axis(ax, pl->precs[PACG_F32])) != PACG_OK) goto fail;
+    if ((rc = split_axis(ax, pl->precs[PACG_F32], d->mfma && d->precision == PACG_PREC_F32)) != PACG_OK) goto fail;
   }
   /* fused axis: requested one if it has stages, else the first that has */
   pl->fused_axis = -1;
@@ -541,7 +568,7 @@ int pacg_plan_autotune(pacg_plan* pl, const void* xr, const void* xi, void* yr, 
       for (int j = 0; j < pacg_variant_counts[t] && !rc; ++j) {
         const pacg_variant_t* v = &pacg_variant_tables[t][j];
         if (v->prec != cur->prec || v->kind != cur->kind || v->layout != cur->layout || v->nstages != cur->nstages ||
-            v->stage_code != cur->stage_code || v->spec_prec != cur->spec_prec)
+            v->stage_code != cur->stage_code || v->spec_prec != cur->spec_prec || v->engine != cur->engine)
           continue;
         ps->v[prec][kind][vsp] = v;
         const void* s_re = kind == PACG_K_FUSED ? pl->spec_re : NULL;

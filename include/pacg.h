@@ -66,6 +66,8 @@ typedef struct {
   int precision;                          /* PACG_PREC_F64 or PACG_PREC_F32: data arrays */
   int spectrum_precision;                 /* storage of A^T k (always computed in FP64) */
   int fused_axis;                         /* axis whose last pass carries the product; -1 = 0 */
+  int mfma;                               /* FP32 plans: run radix-8 stages as F_8 (x) I contractions
+                                             on the matrix cores (split FP16, FP32 accumulation) */
 } pacg_desc;
 
 typedef struct pacg_plan pacg_plan;

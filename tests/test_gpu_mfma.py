@@ -1,0 +1,77 @@
+"""Radix-8 stages on the matrix cores (F_8 (x) I, split FP16, FP32 accumulation)
+against the FP64 oracle.  FP32 bar: rel-L2 <= 2e-6."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+from conftest import rel_l2
+from oracle import A, CONJ, TRANS
+
+pytestmark = pytest.mark.gpu
+
+TOL32 = 2e-6
+
+CASES = [
+    ((512,), [[8, 8, 8]]),
+    ((64, 256), [[8, 8], [8, 8, 4]]),
+    ((8, 64, 32), [[8], [8, 8], [4, 8]]),
+    ((4096, 16), [[8, 8, 8, 8], [4, 4]]),
+    ((32, 8), [[4, 8], [8]]),
+]
+
+
+def rnd(seed, shape):
+    g = np.random.default_rng(seed)
+    return g.uniform(-1, 1, shape) + 1j * g.uniform(-1, 1, shape)
+
+
+@pytest.mark.parametrize("dims,scheds", CASES)
+@pytest.mark.parametrize("sprec", [0, 1])
+def test_mfma_convolution(gpu, oracle, dims, scheds, sprec):
+    x = rnd(31, dims)
+    k = rnd(32, dims)
+    y_ref, _ = oracle.pa_convolution(x, k, scheds)
+    pl = gpu.Plan(dims, schedules=scheds, precision=gpu.PREC_F32, spectrum_precision=sprec, mfma=True)
+    assert "mfma" in pl.describe()
+    pl.set_kernel(k)
+    dx = gpu.DeviceComplex.from_numpy(x, np.float32)
+    dy = pl.alloc()
+    pl.convolve(dx, dy)
+    assert rel_l2(dy.numpy(), y_ref) < TOL32
+
+
+@pytest.mark.parametrize("dims,scheds", CASES)
+def test_mfma_operators(gpu, oracle, dims, scheds):
+    x = rnd(33, dims)
+    pl = gpu.Plan(dims, schedules=scheds, precision=gpu.PREC_F32, mfma=True)
+    for op, kind in ((gpu.OP_TRANS, TRANS), (gpu.OP_CONJ, CONJ), (gpu.OP_FWD, A)):
+        dx = gpu.DeviceComplex.from_numpy(x, np.float32)
+        dy = pl.alloc()
+        pl.execute(op, dx, dy)
+        assert rel_l2(dy.numpy(), oracle.butterflies_nd(x, scheds, kind)) < TOL32, op
+
+
+def test_mfma_block_scaling_handles_wide_range(gpu, oracle):
+    """Values from 1e-30 to 1e30 in one array: per-butterfly scaling keeps FP16 in range."""
+    dims = (64, 64)
+    scheds = [[8, 8], [8, 8]]
+    g = np.random.default_rng(5)
+    x = rnd(34, dims) * 10.0 ** g.integers(-30, 30, dims)
+    pl = gpu.Plan(dims, schedules=scheds, precision=gpu.PREC_F32, mfma=True)
+    dx = gpu.DeviceComplex.from_numpy(x, np.float32)
+    pl.execute(gpu.OP_TRANS, dx)
+    ref = oracle.butterflies_nd(x.astype(np.complex64).astype(np.complex128), scheds, TRANS)
+    assert rel_l2(dx.numpy(), ref) < TOL32
+
+
+def test_anisotropic_4096x256_full_size(gpu, synth):
+    """Config 4: 4096 x 256, U(-1,1), seed 3, radix 8-8-8-8 and 8-8-4, MFMA radix-8 stages."""
+    x, k = synth.config_inputs(4)
+    pl = gpu.Plan(x.shape, schedules=[[8, 8, 8, 8], [8, 8, 4]], precision=gpu.PREC_F32,
+                  spectrum_precision=gpu.PREC_F32, mfma=True)
+    pl.set_kernel(k)
+    dx = gpu.DeviceComplex.from_numpy(x, np.float32)
+    dy = pl.alloc()
+    pl.convolve(dx, dy)
+    ref = np.fft.ifft2(np.fft.fft2(x) * np.fft.fft2(k))
+    assert rel_l2(dy.numpy(), ref) < TOL32
