@@ -70,6 +70,17 @@ def alg_bytes_launch(name, n, e, s):
     return 2 * n * e + (n * s if "_fused_" in name else 0)
 
 
+def measured_traffic(variant):
+    """HBM bytes per launch of `variant` from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs,
+    each counter scaled by what it reads for a copy kernel of known size (profiles/traffic_3d256_fp32.json)."""
+    path = os.path.join(ROOT, "profiles", "traffic_3d256_fp32.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        entry = json.load(f).get("kernels", {}).get(variant)
+    return float(entry["traffic_bytes"]) if entry else None
+
+
 def cpu_baseline(x, k, reps):
     """The reference's permutation_avoiding_convolution_3d (r = 4: 256 = 4^4) on one host core."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -111,6 +122,50 @@ def cpu_baseline(x, k, reps):
             "sample": "256^3 complex FP64 PA convolution, radix 4, 1 warm-up + %d timed runs "
                       "(mean %.3f s +/- %.3f s), h re-copied before each call" % (reps, mean, float(np.std(times))),
             "n": int(n)}
+
+
+def bench_fp64_configs(pacg, synth, steps, warmup):
+    """Configs 1 (1D 2^16, radix 2) and 2 (2D 2048^2, radix 4 with one radix-2 stage), FP64:
+    device-resident plan rate, and the drop-in entry point's rate from host arrays (PCIe included)."""
+    out = {}
+    for cfg, sched_of, name in ((1, lambda n: [2] * 16, "c1_1d_65536_r2"),
+                                (2, lambda n: [4, 4, 4, 4, 4, 2], "c2_2d_2048x2048_r4")):
+        x, k = synth.config_inputs(cfg)
+        n = x.size
+        d = x.ndim
+        scheds = [sched_of(s) for s in x.shape]
+        pl = pacg.Plan(x.shape, schedules=scheds, precision=pacg.PREC_F64)
+        pl.set_kernel(k)
+        dx = pacg.DeviceComplex.from_numpy(x)
+        dy = pl.alloc()
+        pl.autotune(dx, dy, 3)
+        ms = time_convs(pacg, pl, dx, dy, steps, warmup)
+        b_alg = (2 * d - 1) * 2 * n * 16 + n * 16
+        y = dy.numpy()
+        ref = np.fft.ifftn(np.fft.fftn(x) * np.fft.fftn(k))
+        entry = {"ms_per_conv": ms, "conv_per_s": 1000.0 / ms, "launches": pl.launches_per_convolve(),
+                 "alg_bytes": b_alg, "frac": b_alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                 "rel_l2_vs_numpy": float(np.linalg.norm(y - ref) / np.linalg.norm(ref)), "plan": pl.describe()}
+        del pl, dx, dy
+        # drop-in entry point, host arrays in and out (the reference's radix: 2 for both)
+        re0, im0 = np.ascontiguousarray(x.real.ravel()), np.ascontiguousarray(x.imag.ravel())
+        hr0, hi0 = np.ascontiguousarray(k.real.ravel()), np.ascontiguousarray(k.imag.ravel())
+        reps = 5
+        times = []
+        for it in range(reps + 1):
+            re, im, hr, hi = re0.copy(), im0.copy(), hr0.copy(), hi0.copy()
+            t0 = time.perf_counter()
+            if d == 1:
+                pacg.permutation_avoiding_convolution(re, im, n, 2, None, None, hr, hi)
+            else:
+                pacg.permutation_avoiding_convolution_2d(re, im, x.shape[0], x.shape[1], 2, None, None, None, None,
+                                                         hr, hi)
+            if it:
+                times.append(time.perf_counter() - t0)
+        entry["dropin_host_arrays_ms"] = 1000.0 * float(np.mean(times))
+        out[name] = entry
+    pacg.release()
+    return out
 
 
 def bench_mfma(pacg, synth, steps, warmup):
@@ -324,7 +379,7 @@ def main():
                                "spectrum A^T k computed in FP64 on the GPU, stored %s" % a.spectrum,
                    "n": n, "schedule": SCHED, "launches_per_conv": len(prof32)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dom[0]),
                      "kernel": dom[0], "kernel_ms": dom[1], "alg_bytes_per_launch": dom_bytes},
         "roofline_conv": {"alg_bytes": b_alg, "achieved_GBs": conv_gbs, "frac": conv_gbs / HBM_PEAK_GBS},
         "gflops": flops / (ms32 * 1e-3) / 1e9,
@@ -341,6 +396,8 @@ def main():
         "launches_fp32": [{"kernel": nm, "ms": t} for nm, t in prof32],
         "launches_fp64": [{"kernel": nm, "ms": t} for nm, t in prof64],
     }
+    log("fp64 configs 1 and 2 ...")
+    out.update(bench_fp64_configs(pacg, synth, a.steps, a.warmup))
     if not a.no_mfma:
         log("mfma workload ...")
         out["mfma_4096x256"] = bench_mfma(pacg, synth, a.steps, a.warmup)

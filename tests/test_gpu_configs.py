@@ -1,0 +1,79 @@
+"""The benchmark configurations at their full sizes.
+
+Parity at these sizes: against the oracle where it finishes in seconds, else
+against numpy.fft in complex128 (the oracle matches numpy to 1e-15 at the
+sizes test_oracle.py covers), plus size-independent properties."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def test_config1_dropin_1d_65536_r2(gpu, oracle, synth):
+    """1D n = 2^16, radix 2, through the reference-named entry point (host arrays)."""
+    x, k = synth.config_inputs(1)
+    n = x.size
+    y_ref, ath_ref = oracle.pa_convolution(x, k, [[2] * 16])
+    re, im = np.ascontiguousarray(x.real), np.ascontiguousarray(x.imag)
+    hr, hi = np.ascontiguousarray(k.real), np.ascontiguousarray(k.imag)
+    gpu.permutation_avoiding_convolution(re, im, n, 2, None, None, hr, hi)
+    assert rel_l2(re + 1j * im, y_ref) < 1e-12
+    assert rel_l2(hr + 1j * hi, ath_ref) < 1e-12  # h <- A^T h, unscaled, bit-reversed order
+
+
+def test_config2_2048x2048_radix4_fp64(gpu, oracle, synth):
+    """2D 2048^2 with radix-4 stages and one radix-2 on top (2048 = 4^5 * 2), FP64."""
+    x, k = synth.config_inputs(2)
+    sched = [4, 4, 4, 4, 4, 2]
+    pl = gpu.Plan(x.shape, schedules=[sched, sched], precision=gpu.PREC_F64)
+    pl.set_kernel(k)
+    dx = gpu.DeviceComplex.from_numpy(x)
+    dy = pl.alloc()
+    pl.convolve(dx, dy)
+    y = dy.numpy()
+    ref = np.fft.ifft2(np.fft.fft2(x) * np.fft.fft2(k))
+    assert rel_l2(y, ref) < 1e-12
+    # the same schedule against the oracle on a 64-row slab of the problem
+    xs, ks = x[:64], k[:64]
+    pl2 = gpu.Plan(xs.shape, schedules=[[4, 4, 4], sched], precision=gpu.PREC_F64)
+    pl2.set_kernel(ks)
+    d2 = gpu.DeviceComplex.from_numpy(xs)
+    pl2.convolve(d2)
+    y2_ref, _ = oracle.pa_convolution(xs, ks, [[4, 4, 4], sched])
+    assert rel_l2(d2.numpy(), y2_ref) < 1e-12
+
+
+def test_config2_dropin_r2_matches_reference_algorithm(gpu, oracle, synth):
+    """The reference can only run 2048^2 with radix 2 (supports_radix_4(11) is false)."""
+    x, k = synth.config_inputs(2)
+    x, k = x[:256], k[:256]  # 256 x 2048 keeps the oracle fast
+    y_ref, ath_ref = oracle.pa_convolution(x, k, [[2] * 8, [2] * 11])
+    re, im = np.ascontiguousarray(x.real.ravel()), np.ascontiguousarray(x.imag.ravel())
+    hr, hi = np.ascontiguousarray(k.real.ravel()), np.ascontiguousarray(k.imag.ravel())
+    gpu.permutation_avoiding_convolution_2d(re, im, 256, 2048, 2, None, None, None, None, hr, hi)
+    assert rel_l2((re + 1j * im).reshape(x.shape), y_ref) < 1e-12
+    assert rel_l2((hr + 1j * hi).reshape(x.shape), ath_ref) < 1e-12
+
+
+@pytest.mark.parametrize("prec,tol", [(0, 1e-12), (1, 2e-6)])
+def test_config3_256cubed(gpu, synth, prec, tol):
+    """3D 256^3, schedule 8-8-4 per axis, Gaussian stencil (sum 1): FP64 and FP32 against numpy,
+    plus the delta-kernel identity at full size."""
+    x, k = synth.config_inputs(3)
+    scheds = [[8, 8, 4]] * 3
+    pl = gpu.Plan(x.shape, schedules=scheds, precision=prec, spectrum_precision=prec)
+    pl.set_kernel(k)
+    dt = np.float64 if prec == 0 else np.float32
+    dx = gpu.DeviceComplex.from_numpy(x, dt)
+    dy = pl.alloc()
+    pl.convolve(dx, dy)
+    ref = np.fft.ifftn(np.fft.fftn(x) * np.fft.fftn(k))
+    assert rel_l2(dy.numpy(), ref) < tol
+    delta = np.zeros(x.shape, complex)
+    delta[0, 0, 0] = 1.0
+    pl.set_kernel(delta)
+    pl.convolve(dx, dy)
+    assert rel_l2(dy.numpy(), x) < tol
