@@ -1,8 +1,8 @@
 """Benchmark: permutation-avoiding 3D convolution, 256^3 complex, on one MI355X.
 
 Workload (one "step" = one convolution y = (1/n) conj(A)[(A^T k) .* (A^T x)]):
-  3D 256^3 complex, radix-8 schedule per axis (stages 8, 8, 4), bulk butterflies
-  in FP32, twiddles generated in FP64 on the GPU and rounded, kernel spectrum
+  3D 256^3 complex, radix-8 schedule per axis (stages 4, 8, 8), bulk butterflies
+  in FP32 (five launches: one axis per pass, the middle one fused with the product), twiddles generated in FP64 on the GPU and rounded, kernel spectrum
   A^T k computed in FP64 on the GPU; x re/im i.i.d. N(0,1) and k a separable
   Gaussian stencil (sigma = 3 cells), seed 2.  Inputs are resident in HBM
   when the timed region starts; the spectrum is computed once, before it.
@@ -32,7 +32,13 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 PKG_DIR = os.path.join(ROOT, "c-permutation-avoiding-convolution_amd")
 HBM_PEAK_GBS = 8000.0
 N1 = 256
-SCHED = [8, 8, 4]
+SCHED = [4, 8, 8]
+SCHEDS32 = [SCHED] * 3
+SCHEDS64 = [SCHED] * 3
+# pair launches (two axis passes per kernel, 3 launches per convolution), reported beside the
+# headline: the middle axis as 4-8-8 so that its radix-4 bottom stage pairs with axis 0 and its
+# 8-8 top with axis 2
+SCHEDS_PAIRS = [[8, 8, 4], [4, 8, 8], [8, 8, 4]]
 
 
 def log(*a):
@@ -68,6 +74,12 @@ def profile_launches(pacg, pl, dx, dy, reps):
 def alg_bytes_launch(name, n, e, s):
     """Algorithmic HBM bytes of one pass launch: read + write the array, plus the spectrum if fused."""
     return 2 * n * e + (n * s if "_fused_" in name else 0)
+
+
+def conv_alg_bytes(launches, n, e, s):
+    """Algorithmic HBM bytes of one convolution run as `launches` read-modify-write launches of the
+    array, one of which also reads the spectrum."""
+    return launches * 2 * n * e + n * s
 
 
 def measured_traffic(variant):
@@ -261,6 +273,8 @@ def main():
     ap.add_argument("--spectrum", choices=["f32", "f64"], default="f32",
                     help="storage of the FP64-computed spectrum on the headline FP32 path")
     ap.add_argument("--no-tune", action="store_true", help="skip the tile autotuner")
+    ap.add_argument("--pairs", action="store_true",
+                    help="--profile: the pair-launch plan instead of one axis per launch")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched uneven-length workload")
     ap.add_argument("--no-mfma", action="store_true", help="skip the 4096x256 MFMA workload")
     ap.add_argument("--profile-mfma", action="store_true",
@@ -294,12 +308,13 @@ def main():
     x, k = synth.config_inputs(3)
     n = x.size
     log("inputs ready (%.1f s)" % (time.time() - t0))
-    scheds = [SCHED] * 3
+    scheds = SCHEDS32
     sprec = pacg.PREC_F32 if a.spectrum == "f32" else pacg.PREC_F64
     s_bytes = 8 if a.spectrum == "f32" else 16
 
     # ---- FP32 path (headline) -------------------------------------------------
-    pl32 = pacg.Plan([N1] * 3, schedules=scheds, precision=pacg.PREC_F32, spectrum_precision=sprec)
+    pl32 = pacg.Plan([N1] * 3, schedules=SCHEDS_PAIRS if (a.profile and a.pairs) else scheds,
+                     precision=pacg.PREC_F32, spectrum_precision=sprec, pairs=a.profile and a.pairs)
     pl32.set_kernel(k)
     dx32 = pacg.DeviceComplex.from_numpy(x, np.float32)
     dy32 = pl32.alloc()
@@ -320,6 +335,16 @@ def main():
     log("fp32: %.3f ms/conv  plan: %s" % (ms32, pl32.describe()))
     prof32 = profile_launches(pacg, pl32, dx32, dy32, max(5, a.steps // 2))
     y32 = dy32.numpy()
+    nl32 = len(prof32)
+
+    # ---- FP32 with pair launches (two axis passes per kernel) ---------------------------
+    pl1 = pacg.Plan([N1] * 3, schedules=SCHEDS_PAIRS, precision=pacg.PREC_F32, spectrum_precision=sprec, pairs=True)
+    pl1.set_kernel(k)
+    ms_pairs = time_convs(pacg, pl1, dx32, dy32, a.steps, a.warmup)
+    nl_pairs = pl1.launches_per_convolve()
+    prof_pairs = profile_launches(pacg, pl1, dx32, dy32, max(5, a.steps // 4))
+    y_pairs = dy32.numpy()
+    del pl1
 
     # ---- FP32 path with the other spectrum storage --------------------------------
     alt = "f64" if a.spectrum == "f32" else "f32"
@@ -333,13 +358,14 @@ def main():
     del pla
 
     # ---- plain FP64 path (denominator) ------------------------------------------
-    pl64 = pacg.Plan([N1] * 3, schedules=scheds, precision=pacg.PREC_F64)
+    pl64 = pacg.Plan([N1] * 3, schedules=SCHEDS64, precision=pacg.PREC_F64)
     pl64.set_kernel(k)
     dx64 = pacg.DeviceComplex.from_numpy(x, np.float64)
     dy64 = pl64.alloc()
     tuned64 = 0 if a.no_tune else pl64.autotune(dx64, dy64, 3)
     ms64 = time_convs(pacg, pl64, dx64, dy64, max(5, a.steps // 2), a.warmup)
     prof64 = profile_launches(pacg, pl64, dx64, dy64, max(5, a.steps // 4))
+    nl64 = len(prof64)
     y64 = dy64.numpy()
     log("fp64: %.3f ms/conv" % ms64)
     del dx64, dy64, pl64
@@ -351,14 +377,18 @@ def main():
     max32 = float(np.abs(y32 - yref).max())
     err64 = float(np.linalg.norm(y64 - yref) / nref)
     err_alt = float(np.linalg.norm(ya - yref) / nref)
+    err_pairs = float(np.linalg.norm(y_pairs - yref) / nref)
     log("rel-L2 fp32 %.3e  fp64 %.3e" % (err32, err64))
 
     # ---- roofline ------------------------------------------------------------
     dom = max(prof32, key=lambda p: p[1])
     dom_bytes = alg_bytes_launch(dom[0], n, 8, s_bytes)
     achieved = dom_bytes / (dom[1] * 1e-3) / 1e9
-    b_alg = 5 * 2 * n * 8 + n * s_bytes  # (2d-1) round trips of x plus one spectrum read
-    b_alg64 = 5 * 2 * n * 16 + n * 16
+    # round trips of the array actually made (2d-1 = 5 one axis per launch, 3 with pair launches)
+    # plus one spectrum read
+    b_alg = conv_alg_bytes(nl32, n, 8, s_bytes)
+    b_pairs = conv_alg_bytes(nl_pairs, n, 8, s_bytes)
+    b_alg64 = conv_alg_bytes(nl64, n, 16, 16)
     conv_gbs = b_alg / (ms32 * 1e-3) / 1e9
     flops = 2 * 5 * n * np.log2(n)  # two transforms (forward x, inverse), 5 N log2 N each
 
@@ -375,21 +405,26 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded SplitMix64: x ~ N(0,1), k = separable Gaussian sigma 3)",
-        "config": {"workload": "3D 256x256x256 complex PA convolution, radix 8-8-4 per axis, FP32 butterflies, "
-                               "spectrum A^T k computed in FP64 on the GPU, stored %s" % a.spectrum,
-                   "n": n, "schedule": SCHED, "launches_per_conv": len(prof32)},
+        "config": {"workload": "3D 256x256x256 complex PA convolution, radix-8 schedule 4-8-8 per axis, "
+                               "FP32 butterflies, spectrum A^T k computed in FP64 on the GPU, stored %s" % a.spectrum,
+                   "n": n, "schedules": SCHEDS32, "schedules_fp64": SCHEDS64, "launches_per_conv": len(prof32)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dom[0]),
                      "kernel": dom[0], "kernel_ms": dom[1], "alg_bytes_per_launch": dom_bytes},
-        "roofline_conv": {"alg_bytes": b_alg, "achieved_GBs": conv_gbs, "frac": conv_gbs / HBM_PEAK_GBS},
+        "roofline_conv": {"alg_bytes": b_alg, "launches": nl32, "achieved_GBs": conv_gbs,
+                          "frac": conv_gbs / HBM_PEAK_GBS},
+        "fp32_pair_launches": {"schedules": SCHEDS_PAIRS, "ms_per_conv": ms_pairs, "conv_per_s": 1000.0 / ms_pairs,
+                               "launches": nl_pairs, "alg_bytes": b_pairs,
+                               "frac": b_pairs / (ms_pairs * 1e-3) / 1e9 / HBM_PEAK_GBS, "rel_l2": err_pairs,
+                               "kernels": [{"kernel": nm, "ms": t} for nm, t in prof_pairs]},
         "gflops": flops / (ms32 * 1e-3) / 1e9,
-        "fp64_plain": {"ms_per_conv": ms64, "conv_per_s": 1000.0 / ms64, "alg_bytes": b_alg64,
+        "fp64_plain": {"ms_per_conv": ms64, "conv_per_s": 1000.0 / ms64, "launches": nl64, "alg_bytes": b_alg64,
                        "frac": b_alg64 / (ms64 * 1e-3) / 1e9 / HBM_PEAK_GBS, "rel_l2": err64},
         "speedup_vs_fp64": ms64 / ms32,
         "fp32_spectrum_%s" % alt: {"ms_per_conv": ms_alt, "conv_per_s": 1000.0 / ms_alt,
-                                   "alg_bytes": 10 * n * 8 + n * (16 if alt == "f64" else 8),
-                                   "frac": (10 * n * 8 + n * (16 if alt == "f64" else 8)) / (ms_alt * 1e-3) / 1e9
-                                   / HBM_PEAK_GBS,
+                                   "alg_bytes": conv_alg_bytes(nl32, n, 8, 16 if alt == "f64" else 8),
+                                   "frac": conv_alg_bytes(nl32, n, 8, 16 if alt == "f64" else 8)
+                                   / (ms_alt * 1e-3) / 1e9 / HBM_PEAK_GBS,
                                    "speedup_vs_fp64": ms64 / ms_alt, "rel_l2": err_alt},
         "autotuned_launches": {"fp32": tuned32, "fp64": tuned64},
         "error_vs_fp64": {"rel_l2": err32, "max_abs": max32, "reference": "numpy.fft complex128"},

@@ -223,6 +223,17 @@ int pacg_stream_destroy(void* stream) { return (int)hipStreamDestroy((hipStream_
 int pacg_device_sync(void) { return (int)hipDeviceSynchronize(); }
 int pacg_get_device(int* dev) { return (int)hipGetDevice(dev); }
 int pacg_device_count(int* n) { return (int)hipGetDeviceCount(n); }
+int pacg_num_cus(void) {
+  static int cus[64];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+  if (cus[dev] == 0) {
+    int n = 0;
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n < 1) n = 256;
+    cus[dev] = n;
+  }
+  return cus[dev];
+}
 int pacg_last_hip_error(void) { return (int)hipGetLastError(); }
 const char* pacg_hip_error_string(int code) { return hipGetErrorString((hipError_t)code); }
 int pacg_event_create(void** ev) { return (int)hipEventCreate((hipEvent_t*)ev); }

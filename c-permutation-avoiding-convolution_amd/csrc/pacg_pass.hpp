@@ -96,9 +96,11 @@ __device__ __forceinline__ void sfor(F&& f) {
 
 // ---------------------------------------------------------------------------
 // pass shape: NS stages (2 bits each: log2 radix, stage 0 = unit sub-stride),
-// NP register phases (4 bits each: index of the phase's top stage)
+// NP register phases (4 bits each: index of the phase's top stage);
+// LGE raises the points per thread above the largest phase when two passes
+// share one thread block (pair kernels)
 // ---------------------------------------------------------------------------
-template <int NS_, uint32_t SC, int NP_, uint32_t PC>
+template <int NS_, uint32_t SC, int NP_, uint32_t PC, int LGE = 0>
 struct Shape {
   static constexpr int NS = NS_;
   static constexpr int NP = NP_;
@@ -115,7 +117,7 @@ struct Shape {
   static constexpr int plo(int p) { return p == 0 ? 0 : phi(p - 1) + 1; }
   static constexpr int lgR(int p) { return lglam(phi(p) + 1) - lglam(plo(p)); }
   static constexpr int lgE() {
-    int m = 0;
+    int m = LGE;
     for (int p = 0; p < NP; ++p) m = lgR(p) > m ? lgR(p) : m;
     return m;
   }
@@ -139,7 +141,7 @@ struct Shape {
       if (phi(p) < plo(p)) return false;
     for (int g = 0; g < NS; ++g)
       if (lg(g) == 0) return false;
-    return E <= 16 && P <= 16384;
+    return E <= 16 && E <= P && P <= 16384;
   }
 };
 

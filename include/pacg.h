@@ -68,6 +68,10 @@ typedef struct {
   int fused_axis;                         /* axis whose last pass carries the product; -1 = 0 */
   int mfma;                               /* FP32 plans: run radix-8 stages as F_8 (x) I contractions
                                              on the matrix cores (split FP16, FP32 accumulation) */
+  int axis_pairs;                         /* 3D: 1 = run passes of two axes as one launch where a
+                                             pair kernel is compiled (3 round trips per convolution
+                                             instead of 5); 0 = one axis per launch (default: faster
+                                             on MI355X at 256^3, the pair tiles are latency-bound) */
 } pacg_desc;
 
 typedef struct pacg_plan pacg_plan;

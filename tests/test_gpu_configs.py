@@ -58,13 +58,18 @@ def test_config2_dropin_r2_matches_reference_algorithm(gpu, oracle, synth):
     assert rel_l2((hr + 1j * hi).reshape(x.shape), ath_ref) < 1e-12
 
 
-@pytest.mark.parametrize("prec,tol", [(0, 1e-12), (1, 2e-6)])
-def test_config3_256cubed(gpu, synth, prec, tol):
-    """3D 256^3, schedule 8-8-4 per axis, Gaussian stencil (sum 1): FP64 and FP32 against numpy,
-    plus the delta-kernel identity at full size."""
+@pytest.mark.parametrize("prec,tol,scheds,pairs,launches", [
+    (0, 1e-12, [[8, 8, 4]] * 3, False, 5),
+    (1, 2e-6, [[4, 8, 8]] * 3, False, 5),
+    (1, 2e-6, [[8, 8, 4]] * 3, True, 3),
+    (1, 2e-6, [[8, 8, 4], [4, 8, 8], [8, 8, 4]], True, 3),
+])
+def test_config3_256cubed(gpu, synth, prec, tol, scheds, pairs, launches):
+    """3D 256^3, radix-8 schedules, Gaussian stencil (sum 1): FP64 and FP32 against numpy,
+    plus the delta-kernel identity at full size; one axis per launch and pair launches."""
     x, k = synth.config_inputs(3)
-    scheds = [[8, 8, 4]] * 3
-    pl = gpu.Plan(x.shape, schedules=scheds, precision=prec, spectrum_precision=prec)
+    pl = gpu.Plan(x.shape, schedules=scheds, precision=prec, spectrum_precision=prec, pairs=pairs)
+    assert pl.launches_per_convolve() == launches, pl.describe()
     pl.set_kernel(k)
     dt = np.float64 if prec == 0 else np.float32
     dx = gpu.DeviceComplex.from_numpy(x, dt)
