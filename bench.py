@@ -194,15 +194,16 @@ def bench_mfma(pacg, synth, steps, warmup):
         pl.set_kernel(k)
         dx = pacg.DeviceComplex.from_numpy(x, np.float32)
         dy = pl.alloc()
-        if not mf:
-            pl.autotune(dx, dy, 3)
+        pl.autotune(dx, dy, 3)
         ms = time_convs(pacg, pl, dx, dy, steps, warmup)
+        prof = profile_launches(pacg, pl, dx, dy, max(5, steps // 4))
         y = dy.numpy()
         b_alg = 3 * 2 * n * 8 + n * 8
         out[name] = {"ms_per_conv": ms, "conv_per_s": 1000.0 / ms, "launches": pl.launches_per_convolve(),
                      "frac_of_B_alg": b_alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
                      "rel_l2": float(np.linalg.norm(y - ref) / np.linalg.norm(ref)),
-                     "max_abs": float(np.abs(y - ref).max()), "plan": pl.describe()}
+                     "max_abs": float(np.abs(y - ref).max()), "plan": pl.describe(),
+                     "kernels": [{"kernel": nm, "ms": t} for nm, t in prof]}
         del pl, dx, dy
     return out
 
@@ -279,6 +280,8 @@ def main():
     ap.add_argument("--no-mfma", action="store_true", help="skip the 4096x256 MFMA workload")
     ap.add_argument("--profile-mfma", action="store_true",
                     help="only run the 4096x256 MFMA convolution `steps` times (for rocprofv3 counters)")
+    ap.add_argument("--profile-batched", action="store_true",
+                    help="only run the batched workload (graph replay) `steps` times (for rocprofv3)")
     ap.add_argument("--profile", action="store_true",
                     help="only run the FP32 convolution `steps` times after a copy kernel of 2 x 256 MiB "
                          "(counter calibration), for rocprofv3")
@@ -290,6 +293,20 @@ def main():
 
     pacg = load_pacg()
     import pacg.synth as synth
+
+    if a.profile_batched:
+        xs, ks = synth.batched_inputs(4, 4096)
+        ms_ = np.array([int(np.log2(len(x))) for x in xs], dtype=np.int32)
+        b = pacg.Batch(ms_)
+        dx = pacg.DeviceComplex.from_numpy(b.pack(xs), np.float32)
+        dk = pacg.DeviceComplex.from_numpy(b.pack(ks), np.float32)
+        dy = b.alloc()
+        b.capture(dx, dk, dy)
+        for _ in range(a.steps):
+            b.replay()
+        b.sync()
+        print(json.dumps({"profile_batched": True, "steps": a.steps, "points": int(b.total)}))
+        return
 
     if a.profile_mfma:
         x4, k4 = synth.config_inputs(4)

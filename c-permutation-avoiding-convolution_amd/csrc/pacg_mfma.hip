@@ -19,7 +19,8 @@
 // The pass (one read and one write of HBM, like pacg_pass.hpp) stages its
 // tile of lines through LDS: global -> LDS, one MFMA phase per radix-8 stage
 // (twiddles on the VALU, re/im partners exchanged between lanes l and l^32),
-// the pass-external twiddle / spectral product on the VALU, LDS -> global.
+// radix-2/4 stages of the same run on the VALU, the pass-external twiddle /
+// spectral product on the VALU, LDS -> global.
 #include <hip/hip_runtime.h>
 
 #include <stdint.h>
@@ -93,14 +94,35 @@ __device__ __forceinline__ void f8_mfma(const Mat& m, float x[4]) {
 
 }  // namespace mf
 
-// KIND: DIF (A^T), DIT (A), DITC (conj A), FUSED; NS8 radix-8 stages (P = 8^NS8)
-template <typename TS, int NS8, int KIND, int LAYOUT, int LINES>
+// Stage list of an MFMA pass: NS stages, 2 bits each (log2 radix, stage 0 first)
+template <int NS, uint32_t SC>
+struct MShape {
+  static constexpr int lg(int g) { return (int)((SC >> (2 * g)) & 3u); }
+  static constexpr int rad(int g) { return 1 << lg(g); }
+  static constexpr int lglam(int g) {
+    int v = 0;
+    for (int h = 0; h < g; ++h) v += lg(h);
+    return v;
+  }
+  static constexpr int LGP = lglam(NS);
+  static constexpr int P = 1 << LGP;
+  // mixed digit reversal of a point index over the pass's stages
+  static __device__ __forceinline__ int rev(int s) {
+    int out = 0;
+#pragma unroll
+    for (int g = 0; g < NS; ++g) out |= ((s >> lglam(g)) & (rad(g) - 1)) << (LGP - lglam(g + 1));
+    return out;
+  }
+};
+
+// KIND: DIF (A^T), DIT (A), DITC (conj A), FUSED.  Radix-8 stages run on the
+// matrix cores, radix-2/4 stages of the same pass on the VALU, all on one LDS
+// tile of LINES lines.
+template <typename TS, int NS, uint32_t SC, int KIND, int LAYOUT, int LINES>
 __global__ void __launch_bounds__(mf::NT) mfma_pass_kernel(const pacg_pass_args_t a) {
-  constexpr int P = NS8 == 1 ? 8 : 64;
-  constexpr int LDL = P + 1;                 // LDS row per line (+1 against bank conflicts)
-  constexpr int NBF = LINES * (P / 8);       // butterflies per stage in the tile
-  static_assert(NBF % 16 == 0, "whole MFMA groups");
-  constexpr int NGRP = NBF / 16;
+  using MS = MShape<NS, SC>;
+  constexpr int P = MS::P;
+  constexpr int LDL = P + 1;  // LDS row per line (+1 against bank conflicts)
   __shared__ float tre[LINES * LDL], tim[LINES * LDL];
   __shared__ float twr[P], twi[P];
 
@@ -158,40 +180,82 @@ __global__ void __launch_bounds__(mf::NT) mfma_pass_kernel(const pacg_pass_args_
   const bool is_im = q >= 2;
   const int leg0 = 4 * (q & 1);
 
-  // one radix-8 stage g over the tile: DIF (F then twiddle), DIT / DITC (twiddle then F)
-  auto stage = [&](int g, int mode) {
-    const int lam = g == 0 ? 1 : 8;  // sub-stride of stage g within the line
-    for (int grp = wave; grp < NGRP; grp += mf::NT / 64) {
-      const int bf = grp * 16 + jl;            // butterfly id in the tile
-      const int ll = bf / (P / 8), b = bf % (P / 8);
-      const int mu = b % lam, beta = b / lam;
-      const int s0 = beta * (8 * lam) + mu;    // leg m at s0 + m * lam
-      float* arr = is_im ? tim : tre;
-      float x[4];
-      for (int v = 0; v < 4; ++v) x[v] = arr[ll * LDL + s0 + (leg0 + v) * lam];
-      // twiddle w_{8 lam}^{mu m} (index in the w_P table: mu m P / (8 lam))
-      auto twiddle = [&](bool cj) {
-        for (int v = 0; v < 4; ++v) {
-          const float other = __shfl_xor(x[v], 32);  // partner lane: the other component
-          const int idx = (mu * (leg0 + v)) * (P / (8 * lam));
-          const float c = twr[idx];
-          const float sn = cj ? -twi[idx] : twi[idx];
-          const float re = is_im ? other : x[v], im = is_im ? x[v] : other;
-          x[v] = is_im ? (re * sn + im * c) : (re * c - im * sn);
+  // stage G over the tile: DIF (F then twiddle), DIT / DITC (twiddle then F);
+  // leg m of butterfly (mu, beta) at beta * R lam + mu + m lam, twiddle
+  // w_{R lam}^{mu m} = w_P table entry mu m P / (R lam)
+  auto stage = [&](auto G_, int mode) {
+    constexpr int G = decltype(G_)::value;
+    constexpr int R = MS::rad(G);
+    constexpr int lam = 1 << MS::lglam(G);
+    constexpr int NBF = LINES * (P / R);  // butterflies of the stage in the tile
+    if constexpr (R == 8) {
+      static_assert(NBF % 16 == 0, "whole MFMA groups");
+      constexpr int NGRP = NBF / 16;
+      for (int grp = wave; grp < NGRP; grp += mf::NT / 64) {
+        const int bf = grp * 16 + jl;  // butterfly id in the tile
+        const int ll = bf / (P / 8), b = bf % (P / 8);
+        const int mu = b % lam, beta = b / lam;
+        const int s0 = beta * (8 * lam) + mu;
+        float* arr = is_im ? tim : tre;
+        float x[4];
+        for (int v = 0; v < 4; ++v) x[v] = arr[ll * LDL + s0 + (leg0 + v) * lam];
+        auto twiddle = [&](bool cj) {
+          for (int v = 0; v < 4; ++v) {
+            const float other = __shfl_xor(x[v], 32);  // partner lane: the other component
+            const int idx = (mu * (leg0 + v)) * (P / (8 * lam));
+            const float c = twr[idx];
+            const float sn = cj ? -twi[idx] : twi[idx];
+            const float re = is_im ? other : x[v], im = is_im ? x[v] : other;
+            x[v] = is_im ? (re * sn + im * c) : (re * c - im * sn);
+          }
+        };
+        if (mode == M_DIF) {
+          mf::f8_mfma(mfwd, x);
+          if (lam > 1) twiddle(false);
+        } else {
+          if (lam > 1) twiddle(mode == M_DITC);
+          mf::f8_mfma(mode == M_DITC ? minv : mfwd, x);
         }
-      };
-      if (mode == M_DIF) {
-        mf::f8_mfma(mfwd, x);
-        if (lam > 1) twiddle(false);
-      } else {
-        if (lam > 1) twiddle(mode == M_DITC);
-        mf::f8_mfma(mode == M_DITC ? minv : mfwd, x);
+        for (int v = 0; v < 4; ++v) arr[ll * LDL + s0 + (leg0 + v) * lam] = x[v];
       }
-      for (int v = 0; v < 4; ++v) arr[ll * LDL + s0 + (leg0 + v) * lam] = x[v];
+    } else {
+      for (int bf = tid; bf < NBF; bf += mf::NT) {
+        const int ll = bf / (P / R), b = bf % (P / R);
+        const int mu = b % lam, beta = b / lam;
+        const int s0 = ll * LDL + beta * (R * lam) + mu;
+        float xr[R], xi[R];
+#pragma unroll
+        for (int m = 0; m < R; ++m) {
+          xr[m] = tre[s0 + m * lam];
+          xi[m] = tim[s0 + m * lam];
+        }
+        auto twiddle = [&](bool cj) {
+#pragma unroll
+          for (int m = 1; m < R; ++m) {
+            const int idx = (mu * m) * (P / (R * lam));
+            cmul<float>(xr[m], xi[m], twr[idx], cj ? -twi[idx] : twi[idx]);
+          }
+        };
+        if (mode == M_DIF) {
+          dft<float, R, false, 0, 1>(xr, xi);
+          if (lam > 1) twiddle(false);
+        } else {
+          if (lam > 1) twiddle(mode == M_DITC);
+          if (mode == M_DITC)
+            dft<float, R, true, 0, 1>(xr, xi);
+          else
+            dft<float, R, false, 0, 1>(xr, xi);
+        }
+#pragma unroll
+        for (int m = 0; m < R; ++m) {
+          tre[s0 + m * lam] = xr[m];
+          tim[s0 + m * lam] = xi[m];
+        }
+      }
     }
   };
 
-  // element-wise step over the tile: pass-external twiddle, spectral product, final scale
+  // element-wise step over the tile: pass-external twiddle, spectral product
   auto elementwise = [&](int what) {
     for (int idx = tid; idx < LINES * P; idx += mf::NT) {
       const int ll = idx / P, s = idx % P;
@@ -200,9 +264,7 @@ __global__ void __launch_bounds__(mf::NT) mfma_pass_kernel(const pacg_pass_args_
       const uint32_t base = line_base(ll, ok, j);
       float vr = tre[ll * LDL + s], vi = tim[ll * LDL + s];
       if (what == 0 || what == 1) {  // external twiddle w_K^{j' k(s)} (conj when what == 1)
-        int ks = 0;  // digit reversal of s over the pass's radix-8 stages
-        ks = NS8 == 1 ? s : ((s & 7) << 3) | (s >> 3);
-        const uint32_t e = ((j >> a.log2_inner) * (uint32_t)ks) << a.ext_shift;
+        const uint32_t e = ((j >> a.log2_inner) * (uint32_t)MS::rev(s)) << a.ext_shift;
         const float2 h = reinterpret_cast<const float2*>(a.tw_hi)[e >> a.lo_bits];
         const float2 l = reinterpret_cast<const float2*>(a.tw_lo)[e & ((1u << a.lo_bits) - 1u)];
         float wr = h.x * l.x - h.y * l.y, wi = h.x * l.y + h.y * l.x;
@@ -223,10 +285,10 @@ __global__ void __launch_bounds__(mf::NT) mfma_pass_kernel(const pacg_pass_args_
   };
 
   if (KIND == PACG_K_DIF || KIND == PACG_K_FUSED) {
-    for (int g = NS8 - 1; g >= 0; --g) {
-      stage(g, M_DIF);
+    sfor<NS>([&](auto i_) {
+      stage(std::integral_constant<int, NS - 1 - decltype(i_)::value>{}, M_DIF);
       __syncthreads();
-    }
+    });
     if (KIND == PACG_K_DIF && a.ext) {
       elementwise(0);
       __syncthreads();
@@ -234,10 +296,10 @@ __global__ void __launch_bounds__(mf::NT) mfma_pass_kernel(const pacg_pass_args_
     if (KIND == PACG_K_FUSED) {
       elementwise(2);
       __syncthreads();
-      for (int g = 0; g < NS8; ++g) {
-        stage(g, M_DITC);
+      sfor<NS>([&](auto i_) {
+        stage(i_, M_DITC);
         __syncthreads();
-      }
+      });
     }
   } else {
     const int mode = KIND == PACG_K_DITC ? M_DITC : M_DIT;
@@ -245,10 +307,10 @@ __global__ void __launch_bounds__(mf::NT) mfma_pass_kernel(const pacg_pass_args_
       elementwise(mode == M_DITC ? 1 : 0);
       __syncthreads();
     }
-    for (int g = 0; g < NS8; ++g) {
-      stage(g, mode);
+    sfor<NS>([&](auto i_) {
+      stage(i_, mode);
       __syncthreads();
-    }
+    });
   }
 
   // LDS -> global
@@ -272,12 +334,12 @@ __global__ void __launch_bounds__(mf::NT) mfma_pass_kernel(const pacg_pass_args_
   }
 }
 
-template <typename TS, int NS8, int KIND, int LAYOUT, int LINES>
+template <typename TS, int NS, uint32_t SC, int KIND, int LAYOUT, int LINES>
 int launch_mfma(const pacg_pass_args_t* a, void* stream) {
   const uint64_t lines = LAYOUT == PACG_L_STRIDED ? ((uint64_t)a->n_o << a->log2_S) : (uint64_t)a->n_o;
   const uint32_t blocks = (uint32_t)((lines + LINES - 1) / LINES);
   if (blocks == 0) return 0;
-  hipLaunchKernelGGL((mfma_pass_kernel<TS, NS8, KIND, LAYOUT, LINES>), dim3(blocks), dim3(mf::NT), 0,
+  hipLaunchKernelGGL((mfma_pass_kernel<TS, NS, SC, KIND, LAYOUT, LINES>), dim3(blocks), dim3(mf::NT), 0,
                      (hipStream_t)stream, *a);
   return (int)hipGetLastError();
 }
@@ -286,33 +348,43 @@ int launch_mfma(const pacg_pass_args_t* a, void* stream) {
 
 using namespace pacg;
 
-#define MF_ROW(NS8, KIND, LAYOUT, LINES, TS, SP, NAME)                                                        \
-  {PACG_F32, SP, KIND, LAYOUT, NS8, NS8 == 1 ? 0x3u : 0xFu, NS8 == 1 ? 0x0u : 0x10u, NS8 == 1 ? 8 : 64, LINES, \
-   mf::NT, 0, &launch_mfma<TS, NS8, KIND, LAYOUT, LINES>, NAME, 1}
+// one registry row: (stages, stage code, lines per tile, rank, P, kind, layout, spectrum type, name)
+#define MF_ROW(NS, SC, LINES, RANK, P, KIND, LAYOUT, TS, SP, NAME)                                            \
+  {PACG_F32, SP, KIND, LAYOUT, NS, SC, 0u, P, LINES, mf::NT, RANK, &launch_mfma<TS, NS, SC, KIND, LAYOUT, LINES>, \
+   NAME, 1}
+// every kind and layout of one shape and tile (rank 0: the default tile, > 0: autotuner alternatives)
+#define MF_SHAPE(NS, SC, LINES, RANK, P, NM)                                                                   \
+  MF_ROW(NS, SC, LINES, RANK, P, PACG_K_DIF, PACG_L_STRIDED, float, PACG_F32, "mfma_f32_dif_s_" NM),           \
+  MF_ROW(NS, SC, LINES, RANK, P, PACG_K_DIT, PACG_L_STRIDED, float, PACG_F32, "mfma_f32_dit_s_" NM),           \
+  MF_ROW(NS, SC, LINES, RANK, P, PACG_K_DITC, PACG_L_STRIDED, float, PACG_F32, "mfma_f32_ditc_s_" NM),         \
+  MF_ROW(NS, SC, LINES, RANK, P, PACG_K_FUSED, PACG_L_STRIDED, float, PACG_F32, "mfma_f32_fused_s_" NM "_h"),  \
+  MF_ROW(NS, SC, LINES, RANK, P, PACG_K_FUSED, PACG_L_STRIDED, double, PACG_F64, "mfma_f32_fused_s_" NM "_h64"), \
+  MF_ROW(NS, SC, LINES, RANK, P, PACG_K_DIF, PACG_L_CONTIG, float, PACG_F32, "mfma_f32_dif_c_" NM),            \
+  MF_ROW(NS, SC, LINES, RANK, P, PACG_K_DIT, PACG_L_CONTIG, float, PACG_F32, "mfma_f32_dit_c_" NM),            \
+  MF_ROW(NS, SC, LINES, RANK, P, PACG_K_DITC, PACG_L_CONTIG, float, PACG_F32, "mfma_f32_ditc_c_" NM),          \
+  MF_ROW(NS, SC, LINES, RANK, P, PACG_K_FUSED, PACG_L_CONTIG, float, PACG_F32, "mfma_f32_fused_c_" NM "_h"),   \
+  MF_ROW(NS, SC, LINES, RANK, P, PACG_K_FUSED, PACG_L_CONTIG, double, PACG_F64, "mfma_f32_fused_c_" NM "_h64")
 
 extern "C" {
 extern pacg_variant_t pacg_mfma_variants[];
+// runs that hold at least one radix-8 stage, up to 512 points (stage codes: 2 bits per
+// stage, log2 radix, stage 0 lowest); tiles of at most 4096 points keep four or more
+// workgroups resident per CU
 pacg_variant_t pacg_mfma_variants[] = {
-    MF_ROW(2, PACG_K_DIF, PACG_L_STRIDED, 32, float, PACG_F32, "mfma_f32_dif_s_8x8"),
-    MF_ROW(2, PACG_K_DIT, PACG_L_STRIDED, 32, float, PACG_F32, "mfma_f32_dit_s_8x8"),
-    MF_ROW(2, PACG_K_DITC, PACG_L_STRIDED, 32, float, PACG_F32, "mfma_f32_ditc_s_8x8"),
-    MF_ROW(2, PACG_K_FUSED, PACG_L_STRIDED, 32, float, PACG_F32, "mfma_f32_fused_s_8x8_h"),
-    MF_ROW(2, PACG_K_FUSED, PACG_L_STRIDED, 32, double, PACG_F64, "mfma_f32_fused_s_8x8_h64"),
-    MF_ROW(2, PACG_K_DIF, PACG_L_CONTIG, 32, float, PACG_F32, "mfma_f32_dif_c_8x8"),
-    MF_ROW(2, PACG_K_DIT, PACG_L_CONTIG, 32, float, PACG_F32, "mfma_f32_dit_c_8x8"),
-    MF_ROW(2, PACG_K_DITC, PACG_L_CONTIG, 32, float, PACG_F32, "mfma_f32_ditc_c_8x8"),
-    MF_ROW(2, PACG_K_FUSED, PACG_L_CONTIG, 32, float, PACG_F32, "mfma_f32_fused_c_8x8_h"),
-    MF_ROW(2, PACG_K_FUSED, PACG_L_CONTIG, 32, double, PACG_F64, "mfma_f32_fused_c_8x8_h64"),
-    MF_ROW(1, PACG_K_DIF, PACG_L_STRIDED, 128, float, PACG_F32, "mfma_f32_dif_s_8"),
-    MF_ROW(1, PACG_K_DIT, PACG_L_STRIDED, 128, float, PACG_F32, "mfma_f32_dit_s_8"),
-    MF_ROW(1, PACG_K_DITC, PACG_L_STRIDED, 128, float, PACG_F32, "mfma_f32_ditc_s_8"),
-    MF_ROW(1, PACG_K_FUSED, PACG_L_STRIDED, 128, float, PACG_F32, "mfma_f32_fused_s_8_h"),
-    MF_ROW(1, PACG_K_FUSED, PACG_L_STRIDED, 128, double, PACG_F64, "mfma_f32_fused_s_8_h64"),
-    MF_ROW(1, PACG_K_DIF, PACG_L_CONTIG, 128, float, PACG_F32, "mfma_f32_dif_c_8"),
-    MF_ROW(1, PACG_K_DIT, PACG_L_CONTIG, 128, float, PACG_F32, "mfma_f32_dit_c_8"),
-    MF_ROW(1, PACG_K_DITC, PACG_L_CONTIG, 128, float, PACG_F32, "mfma_f32_ditc_c_8"),
-    MF_ROW(1, PACG_K_FUSED, PACG_L_CONTIG, 128, float, PACG_F32, "mfma_f32_fused_c_8_h"),
-    MF_ROW(1, PACG_K_FUSED, PACG_L_CONTIG, 128, double, PACG_F64, "mfma_f32_fused_c_8_h64"),
+    MF_SHAPE(1, 0x3u, 128, 0, 8, "8_l128"),
+    MF_SHAPE(1, 0x3u, 64, 1, 8, "8_l64"),
+    MF_SHAPE(2, 0xFu, 32, 0, 64, "8x8_l32"),
+    MF_SHAPE(2, 0xFu, 16, 1, 64, "8x8_l16"),
+    MF_SHAPE(3, 0x3Fu, 8, 0, 512, "8x8x8_l8"),
+    MF_SHAPE(3, 0x3Fu, 4, 1, 512, "8x8x8_l4"),
+    MF_SHAPE(3, 0x2Fu, 8, 0, 256, "8x8x4_l8"),
+    MF_SHAPE(3, 0x2Fu, 16, 1, 256, "8x8x4_l16"),
+    MF_SHAPE(3, 0x3Eu, 8, 0, 256, "4x8x8_l8"),
+    MF_SHAPE(3, 0x3Eu, 16, 1, 256, "4x8x8_l16"),
+    MF_SHAPE(2, 0xBu, 32, 0, 32, "8x4_l32"),
+    MF_SHAPE(2, 0xEu, 32, 0, 32, "4x8_l32"),
+    MF_SHAPE(2, 0x7u, 64, 0, 16, "8x2_l64"),
+    MF_SHAPE(2, 0xDu, 64, 0, 16, "2x8_l64"),
 };
 extern const int pacg_num_mfma_variants;
 const int pacg_num_mfma_variants = sizeof(pacg_mfma_variants) / sizeof(pacg_mfma_variants[0]);

@@ -126,7 +126,7 @@ static const step_t* steps_for(const pacg_plan* pl, int prec, int* n);
 static const pacg_variant_t* find_mfma(int spec_prec, int kind, int layout, int ns, uint32_t code) {
   for (int i = 0; i < pacg_num_mfma_variants; ++i) {
     const pacg_variant_t* v = &pacg_mfma_variants[i];
-    if (v->kind == kind && v->layout == layout && v->nstages == ns && v->stage_code == code &&
+    if (v->rank == 0 && v->kind == kind && v->layout == layout && v->nstages == ns && v->stage_code == code &&
         (kind != PACG_K_FUSED || v->spec_prec == spec_prec))
       return v;
   }
@@ -204,10 +204,23 @@ static int split_axis(axis_t* ax, int need_f32, int mfma) {
   ax->npass = 0;
   while (q < ax->t) {
     int best = 0, is_mfma = 0;
-    if (mfma && ax->radix[q] == 8) {
-      /* radix-8 stages go to the matrix cores, two per pass */
-      best = (q + 1 < ax->t && ax->radix[q + 1] == 8) ? 2 : 1;
-      is_mfma = 1;
+    if (mfma) {
+      /* the longest run with a radix-8 stage that has an MFMA pass kernel (its
+       * radix-8 stages on the matrix cores, the others on the VALU) */
+      for (int len = ax->t - q; len >= 1 && !best; --len) {
+        int has8 = 0;
+        long long P = 1;
+        for (int i = q; i < q + len; ++i) {
+          has8 |= ax->radix[i] == 8;
+          P *= ax->radix[i];
+        }
+        if (!has8 || P > 512) continue;
+        uint32_t code = run_code(ax->radix, q, q + len - 1);
+        if (!find_mfma(PACG_F32, PACG_K_DIF, PACG_L_STRIDED, len, code)) continue;
+        if (!find_variant(PACG_F64, PACG_F64, PACG_K_DIF, PACG_L_STRIDED, len, code)) continue;
+        best = len;
+        is_mfma = 1;
+      }
     }
     for (int len = ax->t - q; len >= 1 && !best; --len) {
       if (mfma) {
@@ -726,9 +739,12 @@ int pacg_plan_autotune(pacg_plan* pl, const void* xr, const void* xi, void* yr, 
     if (!cur) continue;
     const pacg_variant_t* best = cur;
     float best_ms = 1e30f;
-    for (int t = 0; t < pacg_num_variant_tables && !rc; ++t) {
-      for (int j = 0; j < pacg_variant_counts[t] && !rc; ++j) {
-        const pacg_variant_t* v = &pacg_variant_tables[t][j];
+    /* tables: the generated pass kernels, then the MFMA pass kernels */
+    for (int t = 0; t <= pacg_num_variant_tables && !rc; ++t) {
+      const pacg_variant_t* tab = t < pacg_num_variant_tables ? pacg_variant_tables[t] : pacg_mfma_variants;
+      const int cnt = t < pacg_num_variant_tables ? pacg_variant_counts[t] : pacg_num_mfma_variants;
+      for (int j = 0; j < cnt && !rc; ++j) {
+        const pacg_variant_t* v = &tab[j];
         if (v->prec != cur->prec || v->kind != cur->kind || v->layout != cur->layout || v->nstages != cur->nstages ||
             v->stage_code != cur->stage_code || v->spec_prec != cur->spec_prec || v->engine != cur->engine)
           continue;

@@ -17,7 +17,17 @@ CASES = [
     ((8, 64, 32), [[8], [8, 8], [4, 8]]),
     ((4096, 16), [[8, 8, 8, 8], [4, 4]]),
     ((32, 8), [[4, 8], [8]]),
+    ((256, 64), [[4, 8, 8], [2, 8, 4]]),
+    ((16, 512), [[2, 8], [8, 8, 8]]),
 ]
+
+
+def test_mfma_plan_one_pass_per_axis(gpu):
+    """4096 x 256 (8-8-8-8 / 8-8-4): the MFMA plan cuts the axes like the VALU plan (5 launches)."""
+    scheds = [[8, 8, 8, 8], [8, 8, 4]]
+    mf = gpu.Plan((4096, 256), schedules=scheds, precision=gpu.PREC_F32, mfma=True)
+    va = gpu.Plan((4096, 256), schedules=scheds, precision=gpu.PREC_F32)
+    assert mf.launches_per_convolve() == va.launches_per_convolve() == 5, mf.describe()
 
 
 def rnd(seed, shape):
