@@ -19,6 +19,10 @@ SHAPES = [[2] * k for k in range(1, 9)] + [[4] * k for k in range(1, 5)] + [[8] 
 # ... and the mixed runs of the mixed-radix schedules the planner uses:
 # 256 = 4*8*8 and 8*8*4, 2048 = 4^5 * 2 split as (4^4)(4, 2), 32 = 4*8 / 8*4
 SHAPES += [[4, 8, 8], [8, 8, 4], [4, 2], [2, 4], [4, 8], [8, 4], [8, 2], [2, 8]]
+# whole 2048-point axes in one pass, contiguous lines only (a strided tile of
+# 2048-point lines would not leave room for enough lines in LDS): 4^5 * 2,
+# 2^11 (the reference's radix for 2048) and 8^3 * 4 / 4 * 8^3
+LONG_SHAPES = [[4, 4, 4, 4, 4, 2], [2] * 11, [8, 8, 8, 4], [4, 8, 8, 8]]
 
 E_TARGET = 16
 
@@ -105,13 +109,13 @@ def emit(prec: str, outdir: str):
     pconst = "PACG_F64" if prec == "double" else "PACG_F32"
     spec_types = ["double"] if prec == "double" else ["float", "double"]
     rows = []
-    for shape in SHAPES:
+    for shape in SHAPES + LONG_SHAPES:
         sc, npph, pc = codes(shape)
         P, E, tpl, sl, nl = geometry(shape, prec)
         shname = "Shape<%d, 0x%Xu, %d, 0x%Xu>" % (len(shape), sc, npph, pc)
         sname = "x".join(str(r) for r in shape)
         for kind in KINDS:
-            for layout in LAYOUTS:
+            for layout in (LAYOUTS if shape in SHAPES else ["PACG_L_CONTIG"]):
                 specs = spec_types if kind == "PACG_K_FUSED" else [T]
                 for rank, ln in enumerate(tile_options(shape, prec, layout)):
                     for ts in specs:

@@ -417,7 +417,11 @@ __global__ void __launch_bounds__(LINES* SH::TPL)
   constexpr bool NEED_X = NP > 1;
   __shared__ T xr[NEED_X ? LDS_N : 1];
   __shared__ T xi[NEED_X ? LDS_N : 1];
-  __shared__ T twr[P], twi[P];
+  // w_P in LDS: the whole table up to 512 points, two levels above
+  // (w_P^p = hi[p >> 6] * lo[p & 63]) so that long lines keep LDS for data
+  constexpr bool TW2 = P > 512;
+  constexpr int TWN = TW2 ? P / 64 + 64 : P;
+  __shared__ T twr[TWN], twi[TWN];
 
   const int tid = threadIdx.x;
   int tl, ll;
@@ -446,23 +450,37 @@ __global__ void __launch_bounds__(LINES* SH::TPL)
   }
   const uint32_t stride = (LAYOUT == PACG_L_STRIDED) ? S : 1u;
 
+  T vr[E], vi[E];
+
   // w_P table -> LDS
   {
     using V2 = typename Vec2<T>::type;
     const V2* tp = reinterpret_cast<const V2*>(a.tw_p);
-    for (int i = tid; i < P; i += NT) {
-      V2 w = tp[i];
+    for (int i = tid; i < TWN; i += NT) {
+      const V2 w = tp[TW2 ? (i < P / 64 ? i * 64 : i - P / 64) : i];
       twr[i] = w.x;
       twi[i] = w.y;
     }
   }
+  auto ptw = [&](auto PH_, auto CJ_) PACG_INL {
+    constexpr int PH = decltype(PH_)::value;
+    constexpr bool CJ = decltype(CJ_)::value;
+    if constexpr (TW2) {
+      phase_twiddle_f<T, SH, PH, CJ>(vr, vi, tl, [&](int idx, T& wr, T& wi) PACG_INL {
+        const T hr = twr[idx >> 6], hi = twi[idx >> 6];
+        const T lr = twr[P / 64 + (idx & 63)], li = twi[P / 64 + (idx & 63)];
+        wr = hr * lr - hi * li;
+        wi = hr * li + hi * lr;
+      });
+    } else {
+      phase_twiddle<T, SH, PH, CJ>(vr, vi, twr, twi, tl);
+    }
+  };
 
   const T* __restrict__ inr = reinterpret_cast<const T*>(a.in_re);
   const T* __restrict__ ini = reinterpret_cast<const T*>(a.in_im);
   T* __restrict__ outr = reinterpret_cast<T*>(a.out_re);
   T* __restrict__ outi = reinterpret_cast<T*>(a.out_im);
-
-  T vr[E], vi[E];
 
   auto lds_index = [&](int s) PACG_INL -> int {
     if constexpr (LAYOUT == PACG_L_STRIDED)
@@ -620,7 +638,7 @@ __global__ void __launch_bounds__(LINES* SH::TPL)
       constexpr int PH = NP - 1 - decltype(i_)::value;
       if constexpr (PH < NP - 1) exchange(std::integral_constant<int, PH + 1>{}, std::integral_constant<int, PH>{});
       phase_stages<T, SH, PH, M_DIF>(vr, vi);
-      phase_twiddle<T, SH, PH, false>(vr, vi, twr, twi, tl);
+      ptw(std::integral_constant<int, PH>{}, std::false_type{});
     });
     if constexpr (KIND == PACG_K_DIF) {
       if (a.ext) ext_twiddle(false);
@@ -637,7 +655,7 @@ __global__ void __launch_bounds__(LINES* SH::TPL)
       sfor<NP>([&](auto i_) PACG_INL {
         constexpr int PH = decltype(i_)::value;
         if constexpr (PH > 0) exchange(std::integral_constant<int, PH - 1>{}, std::integral_constant<int, PH>{});
-        phase_twiddle<T, SH, PH, true>(vr, vi, twr, twi, tl);
+        ptw(std::integral_constant<int, PH>{}, std::true_type{});
         phase_stages<T, SH, PH, M_DITC>(vr, vi);
       });
       store_phase(std::integral_constant<int, NP - 1>{}, (T)a.out_scale);
@@ -650,7 +668,7 @@ __global__ void __launch_bounds__(LINES* SH::TPL)
     sfor<NP>([&](auto i_) PACG_INL {
       constexpr int PH = decltype(i_)::value;
       if constexpr (PH > 0) exchange(std::integral_constant<int, PH - 1>{}, std::integral_constant<int, PH>{});
-      phase_twiddle<T, SH, PH, MODE == M_DITC>(vr, vi, twr, twi, tl);
+      ptw(std::integral_constant<int, PH>{}, std::integral_constant<bool, MODE == M_DITC>{});
       phase_stages<T, SH, PH, MODE>(vr, vi);
     });
     store_phase(std::integral_constant<int, NP - 1>{}, (T)a.out_scale);

@@ -198,7 +198,10 @@ static int default_schedule(int N, int* radix) {
 }
 
 /* Cut one axis' stages into passes, bottom up, taking the longest run with a
- * compiled kernel (P <= 512).  Single stages always have one. */
+ * compiled kernel for the layout the pass will have (P <= 512 on strided
+ * lines; up to 2048 on contiguous lines, the bottom pass of the last axis,
+ * when that leaves at least 1024 lines).
+ * Single stages always have one. */
 static int split_axis(axis_t* ax, int need_f32, int mfma) {
   int q = 0;
   ax->npass = 0;
@@ -222,6 +225,10 @@ static int split_axis(axis_t* ax, int need_f32, int mfma) {
         is_mfma = 1;
       }
     }
+    /* this pass' lines are contiguous; passes longer than 512 points only where they
+     * still leave 1024 or more lines to spread over the CUs */
+    const int contig = q == 0 && ax->inner == 1;
+    const long long npts = ax->outer * (long long)ax->N * ax->inner;
     for (int len = ax->t - q; len >= 1 && !best; --len) {
       if (mfma) {
         int has8 = 0;
@@ -231,10 +238,15 @@ static int split_axis(axis_t* ax, int need_f32, int mfma) {
       if (len > 12) continue;
       long long P = 1;
       for (int i = q; i < q + len; ++i) P *= ax->radix[i];
-      if (P > 512) continue;
+      if (P > (contig ? 2048 : 512) || (P > 512 && npts / P < 1024)) continue;
       uint32_t code = run_code(ax->radix, q, q + len - 1);
-      if (!find_variant(PACG_F64, PACG_F64, PACG_K_DIF, PACG_L_STRIDED, len, code)) continue;
-      if (need_f32 && !find_variant(PACG_F32, PACG_F32, PACG_K_DIF, PACG_L_STRIDED, len, code)) continue;
+      int lay = PACG_L_STRIDED;
+      if (!find_variant(PACG_F64, PACG_F64, PACG_K_DIF, lay, len, code)) {
+        if (!contig) continue;
+        lay = PACG_L_CONTIG;
+        if (!find_variant(PACG_F64, PACG_F64, PACG_K_DIF, lay, len, code)) continue;
+      }
+      if (need_f32 && !find_variant(PACG_F32, PACG_F32, PACG_K_DIF, lay, len, code)) continue;
       best = len;
       break;
     }
