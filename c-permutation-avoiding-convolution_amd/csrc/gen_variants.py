@@ -74,7 +74,7 @@ def geometry(shape, prec):
     while P * (strided_lines + 1) * 2 * elem > 80 * 1024 and strided_lines > 4:
         strided_lines //= 2
     nl = max(1, 256 // tpl)
-    ldp = P + P // 32 + 1
+    ldp = P + P // 8 + 1
     while nl > 1 and nl * ldp * 2 * elem > 40 * 1024:
         nl //= 2
     return P, E, tpl, strided_lines, nl
@@ -94,7 +94,7 @@ def tile_options(shape, prec, layout):
         opts = [nl]
         if P >= 64:
             for alt in (nl // 2, nl * 2):
-                if 1 <= alt and alt * tpl <= 1024 and alt * tpl >= 64 and alt * (P + P // 32 + 1) * 2 * elem <= 80 * 1024:
+                if 1 <= alt and alt * tpl <= 1024 and alt * tpl >= 64 and alt * (P + P // 8 + 1) * 2 * elem <= 80 * 1024:
                     opts.append(alt)
     return opts
 

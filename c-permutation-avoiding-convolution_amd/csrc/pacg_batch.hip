@@ -11,7 +11,7 @@
 // 1024 threads takes items blockIdx.x, blockIdx.x + gridDim.x, ... until none
 // are left (every wave reaches the same exit).  Within an item, the n / E
 // threads of one convolution each hold E points per register phase; the
-// phases exchange through LDS (at most 2^14 points, 135 KB with padding).
+// phases exchange through LDS (at most 2^14 points, 147 KB with padding).
 // Schedules per length come from a variant table the autotuner picks from.
 #include <hip/hip_runtime.h>
 
@@ -24,7 +24,7 @@ namespace pacg {
 
 constexpr int BT = 1024;              // threads per workgroup
 constexpr int BPTS = 1 << 14;         // points per item
-constexpr int BLD = BPTS + BPTS / 32; // padded LDS length per array
+constexpr int BLD = BPTS + BPTS / 8;  // padded LDS length per array (one slot per 8 points)
 
 template <class SH>
 __device__ __forceinline__ void batch_item(const pacg_batch_args_t& a, const pacg_batch_item_t it, float* lre,
@@ -32,7 +32,7 @@ __device__ __forceinline__ void batch_item(const pacg_batch_args_t& a, const pac
   constexpr int P = SH::P, E = SH::E, TPL = SH::TPL, NP = SH::NP;
   static_assert(BT % TPL == 0, "threads per convolution");
   constexpr int G = BT / TPL;  // convolution slots per item
-  constexpr int LDP = P + P / 32;
+  constexpr int LDP = P + P / 8;
   // Recompute every per-thread index inside the persistent loop: hoisted out
   // of it, the positions and LDS addresses of all phases would stay live.
   int tid = threadIdx.x;
@@ -50,7 +50,7 @@ __device__ __forceinline__ void batch_item(const pacg_batch_args_t& a, const pac
     wr = h.x * l.x - h.y * l.y;
     wi = h.x * l.y + h.y * l.x;
   };
-  auto li = [&](int s) PACG_INL { return g * LDP + s + (s >> 5); };
+  auto li = [&](int s) PACG_INL { return g * LDP + s + (s >> 3); };
 
   // Spectrum A^T k: in registers, except at 16 points per thread where it is
   // parked in this workgroup's slab of a.spec_slab (each thread re-reads only
@@ -173,8 +173,8 @@ __device__ __forceinline__ void batch_item(const pacg_batch_args_t& a, const pac
   __syncthreads();  // LDS is reused by the next item
 }
 
-// Lines longer than BT * 8 points: each thread runs VT = TPL / BT virtual
-// threads of 8 points in turn, the line stays in LDS between phases (the same
+// Lines of BT * 8 points and more (2^13, 2^14): each thread runs VT = TPL / BT
+// virtual threads of 8 points in turn, the line stays in LDS between phases (the same
 // LDS traffic as the exchange model, half the live registers) and the
 // spectrum is parked in the workgroup's slab.  One convolution per item.
 template <class SH>
@@ -195,7 +195,7 @@ __device__ __forceinline__ void batch_item_vt(const pacg_batch_args_t& a, const 
     wr = h.x * l.x - h.y * l.y;
     wi = h.x * l.y + h.y * l.x;
   };
-  auto li = [&](int s) PACG_INL { return s + (s >> 5); };
+  auto li = [&](int s) PACG_INL { return s + (s >> 3); };
   float2* slab = reinterpret_cast<float2*>(a.spec_slab) + (size_t)blockIdx.x * BPTS;
   const float scale = 1.0f / (float)P;
 

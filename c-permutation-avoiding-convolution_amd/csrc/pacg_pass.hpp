@@ -410,9 +410,11 @@ __global__ void __launch_bounds__(LINES* SH::TPL)
   static_assert(SH::valid(), "invalid pass shape");
   constexpr int P = SH::P, E = SH::E, TPL = SH::TPL, NP = SH::NP;
   constexpr int NT = LINES * TPL;
-  // LDS: strided tiles are [P][LINES+1], contiguous tiles [LINES][P + P/32 + 1]
+  // LDS: strided tiles are [P][LINES+1], contiguous tiles [LINES][P + P/8 + 1]
+  // with one pad slot per 8 points (s + s/8 keeps the phase layouts of radix-4/8
+  // schedules at most 2-way bank conflicted)
   constexpr int LDJ = LINES + 1;
-  constexpr int LDP = P + P / 32 + 1;
+  constexpr int LDP = P + P / 8 + 1;
   constexpr int LDS_N = (LAYOUT == PACG_L_STRIDED) ? P * LDJ : LINES * LDP;
   constexpr bool NEED_X = NP > 1;
   __shared__ T xr[NEED_X ? LDS_N : 1];
@@ -486,7 +488,7 @@ __global__ void __launch_bounds__(LINES* SH::TPL)
     if constexpr (LAYOUT == PACG_L_STRIDED)
       return s * LDJ + ll;
     else
-      return ll * LDP + s + (s >> 5);
+      return ll * LDP + s + (s >> 3);
   };
 
   // Phase 0 of a contiguous line holds R consecutive points per group:
