@@ -26,12 +26,37 @@ constexpr int BT = 1024;              // threads per workgroup
 constexpr int BPTS = 1 << 14;         // points per item
 constexpr int BLD = BPTS + BPTS / 8;  // padded LDS length per array (one slot per 8 points)
 
+// Phase twiddle of phase PH from the two LDS tables of w_{2^14}: the index
+// e = idx << (14 - log2 P) of a phase whose stages span at most 128 points
+// (lglam(top + 1) <= 7) is a multiple of 128, so w = hi[e >> 7] exactly and
+// the second table and the product are skipped (SINGLE; the virtual-thread
+// path keeps the product, where the shorter code spilled registers).
+template <class SH, int PH, bool CJ, bool SINGLE = true>
+__device__ __forceinline__ void batch_phase_twiddle(float* vr, float* vi, int tl, const float2* twh,
+                                                    const float2* twl_) {
+  constexpr int TW_SHIFT = 14 - SH::LGP;
+  if constexpr (SINGLE && SH::lglam(SH::phi(PH) + 1) <= 7) {
+    phase_twiddle_f<float, SH, PH, CJ>(vr, vi, tl, [&](int idx, float& wr, float& wi) PACG_INL {
+      const float2 h = twh[(idx << TW_SHIFT) >> 7];
+      wr = h.x;
+      wi = h.y;
+    });
+  } else {
+    phase_twiddle_f<float, SH, PH, CJ>(vr, vi, tl, [&](int idx, float& wr, float& wi) PACG_INL {
+      const int e = idx << TW_SHIFT;
+      const float2 h = twh[e >> 7];
+      const float2 l = twl_[e & 127];
+      wr = h.x * l.x - h.y * l.y;
+      wi = h.x * l.y + h.y * l.x;
+    });
+  }
+}
+
 template <class SH>
 __device__ __forceinline__ void batch_item(const pacg_batch_args_t& a, const pacg_batch_item_t it, float* lre,
                                            float* lim, const float2* twh, const float2* twl_) {
   constexpr int P = SH::P, E = SH::E, TPL = SH::TPL, NP = SH::NP;
-  static_assert(BT % TPL == 0, "threads per convolution");
-  constexpr int G = BT / TPL;  // convolution slots per item
+  static_assert(BT % TPL == 0, "threads per convolution (BT / TPL convolutions per item)");
   constexpr int LDP = P + P / 8;
   // Recompute every per-thread index inside the persistent loop: hoisted out
   // of it, the positions and LDS addresses of all phases would stay live.
@@ -41,15 +66,7 @@ __device__ __forceinline__ void batch_item(const pacg_batch_args_t& a, const pac
   const int tl = tid % TPL;
   const bool valid = g < (int)it.count;
   const uint32_t base = valid ? a.conv_off[it.first + g] : 0u;
-  // w_P^idx = w_{2^14}^e, e = idx << TW_SHIFT = hi[e >> 7] * lo[e & 127] (two LDS tables)
-  constexpr int TW_SHIFT = 14 - SH::LGP;
-  auto twl = [&](int idx, float& wr, float& wi) PACG_INL {
-    const int e = idx << TW_SHIFT;
-    const float2 h = twh[e >> 7];
-    const float2 l = twl_[e & 127];
-    wr = h.x * l.x - h.y * l.y;
-    wi = h.x * l.y + h.y * l.x;
-  };
+  // phase twiddles: w_P^idx = w_{2^14}^e from two LDS tables (batch_phase_twiddle)
   auto li = [&](int s) PACG_INL { return g * LDP + s + (s >> 3); };
 
   // Spectrum A^T k: in registers, except at 16 points per thread where it is
@@ -105,7 +122,7 @@ __device__ __forceinline__ void batch_item(const pacg_batch_args_t& a, const pac
       constexpr int PH = NP - 1 - decltype(i_)::value;
       if constexpr (PH < NP - 1) exchange(std::integral_constant<int, PH + 1>{}, std::integral_constant<int, PH>{});
       phase_stages<float, SH, PH, M_DIF>(vr, vi);
-      phase_twiddle_f<float, SH, PH, false>(vr, vi, tl, twl);
+      batch_phase_twiddle<SH, PH, false>(vr, vi, tl, twh, twl_);
     });
   };
 
@@ -150,7 +167,7 @@ __device__ __forceinline__ void batch_item(const pacg_batch_args_t& a, const pac
   sfor<NP>([&](auto i_) PACG_INL {
     constexpr int PH = decltype(i_)::value;
     if constexpr (PH > 0) exchange(std::integral_constant<int, PH - 1>{}, std::integral_constant<int, PH>{});
-    phase_twiddle_f<float, SH, PH, true>(vr, vi, tl, twl);
+    batch_phase_twiddle<SH, PH, true>(vr, vi, tl, twh, twl_);
     phase_stages<float, SH, PH, M_DITC>(vr, vi);
   });
   {
@@ -187,14 +204,6 @@ __device__ __forceinline__ void batch_item_vt(const pacg_batch_args_t& a, const 
   asm volatile("" : "+v"(tid));
   const bool valid = it.count > 0;
   const uint32_t base = valid ? a.conv_off[it.first] : 0u;
-  constexpr int TW_SHIFT = 14 - SH::LGP;
-  auto twl = [&](int idx, float& wr, float& wi) PACG_INL {
-    const int e = idx << TW_SHIFT;
-    const float2 h = twh[e >> 7];
-    const float2 l = twl_[e & 127];
-    wr = h.x * l.x - h.y * l.y;
-    wi = h.x * l.y + h.y * l.x;
-  };
   auto li = [&](int s) PACG_INL { return s + (s >> 3); };
   float2* slab = reinterpret_cast<float2*>(a.spec_slab) + (size_t)blockIdx.x * BPTS;
   const float scale = 1.0f / (float)P;
@@ -224,9 +233,9 @@ __device__ __forceinline__ void batch_item_vt(const pacg_batch_args_t& a, const 
     });
     if constexpr (MODE == M_DIF) {
       phase_stages<float, SH, PH, M_DIF>(vr, vi);
-      phase_twiddle_f<float, SH, PH, false>(vr, vi, vtl, twl);
+      batch_phase_twiddle<SH, PH, false, false>(vr, vi, vtl, twh, twl_);
     } else {
-      phase_twiddle_f<float, SH, PH, true>(vr, vi, vtl, twl);
+      batch_phase_twiddle<SH, PH, true, false>(vr, vi, vtl, twh, twl_);
       phase_stages<float, SH, PH, M_DITC>(vr, vi);
     }
     // tail: 0 write LDS, 1 k spectrum -> slab, 2 x: product + conj(A) phase 0 -> LDS, 3 -> y
