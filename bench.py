@@ -315,6 +315,8 @@ def main():
         pl.set_kernel(k4)
         dx = pacg.DeviceComplex.from_numpy(x4, np.float32)
         dy = pl.alloc()
+        pl.autotune(dx, dy, 3)  # the tiles the bench line reports
+        pl.sync()
         for _ in range(a.steps):
             pl.convolve(dx, dy)
         pl.sync()

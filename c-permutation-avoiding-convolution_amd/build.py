@@ -5,7 +5,9 @@
 1. gen_variants.py writes the pass-kernel instantiation units into csrc/gen/;
 2. every .hip unit is compiled by hipcc for gfx950 (parallel jobs);
 3. the C host layer (plans, drop-in fft.h entry points, helpers) by gcc;
-4. hipcc links the shared library lib/libpacg.so.
+4. hipcc links the shared library lib/libpacg.so;
+5. gcc builds the verify/bench driver programs into bin/ from drivers/bench_driver.c,
+   linked against lib/libpacg.so through the fft.h names.
 --incremental skips objects newer than their sources (developer loop only).
 """
 from __future__ import annotations
@@ -23,6 +25,22 @@ CSRC = os.path.join(PKG, "csrc")
 GEN = os.path.join(CSRC, "gen")
 OBJ = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "lib")
+BIN = os.path.join(PKG, "bin")
+DRIVER_SRC = os.path.join(PKG, "drivers", "bench_driver.c")
+# program name -> defines; the names and command lines of the reference's driver programs
+DRIVERS = {
+    "bench1d_convolution": ["-DPACG_DIM=1", "-DPACG_KIND=PACG_CONV"],
+    "bench1d_forward_fft": ["-DPACG_DIM=1", "-DPACG_KIND=PACG_FWD"],
+    "bench1d_backward_fft": ["-DPACG_DIM=1", "-DPACG_KIND=PACG_BWD"],
+    "bench2d_convolution": ["-DPACG_DIM=2", "-DPACG_KIND=PACG_CONV"],
+    "bench2d_convolution_anisotropic": ["-DPACG_DIM=2", "-DPACG_KIND=PACG_CONV", "-DPACG_ANISO"],
+    "bench2d_forward_fft": ["-DPACG_DIM=2", "-DPACG_KIND=PACG_FWD"],
+    "bench2d_backward_fft": ["-DPACG_DIM=2", "-DPACG_KIND=PACG_BWD"],
+    "bench3d_convolution": ["-DPACG_DIM=3", "-DPACG_KIND=PACG_CONV"],
+    "bench3d_convolution_anisotropic": ["-DPACG_DIM=3", "-DPACG_KIND=PACG_CONV", "-DPACG_ANISO"],
+    "bench3d_forward_fft": ["-DPACG_DIM=3", "-DPACG_KIND=PACG_FWD"],
+    "bench3d_backward_fft": ["-DPACG_DIM=3", "-DPACG_KIND=PACG_BWD"],
+}
 INCLUDE = os.path.join(ROOT, "include")
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
@@ -85,9 +103,22 @@ def build(jobs: int | None = None, incremental: bool = False, verbose: bool = Tr
     tmp = out + ".tmp"
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs, "-lpthread", "-lm"])
     os.replace(tmp, out)
+    build_drivers(jobs)
     if verbose:
-        print("built %s (%d units compiled)" % (out, len(tasks)))
+        print("built %s (%d units compiled) and %d driver programs in %s" % (out, len(tasks), len(DRIVERS), BIN))
     return out
+
+
+def build_drivers(jobs: int | None = None) -> None:
+    """The verify/bench programs; each finds lib/libpacg.so through its rpath."""
+    os.makedirs(BIN, exist_ok=True)
+    tasks = []
+    for name, defs in DRIVERS.items():
+        tasks.append(["gcc", "-O2", "-std=c11", "-Wall", "-Wextra", "-Werror", "-I", INCLUDE, *defs, DRIVER_SRC,
+                      "-o", os.path.join(BIN, name), "-L", LIB, "-lpacg", "-Wl,-rpath,$ORIGIN/../lib", "-lm"])
+    with cf.ThreadPoolExecutor(max_workers=jobs or min(16, os.cpu_count() or 4)) as ex:
+        for f in cf.as_completed([ex.submit(_run, t) for t in tasks]):
+            f.result()
 
 
 if __name__ == "__main__":
