@@ -11,7 +11,8 @@
 // 1024 threads takes items blockIdx.x, blockIdx.x + gridDim.x, ... until none
 // are left (every wave reaches the same exit).  Within an item, the n / E
 // threads of one convolution each hold E points per register phase; the
-// phases exchange through LDS (at most 2^14 points, 147 KB with padding).
+// phases exchange through LDS (at most 2^14 points, 128 KB; slots swizzled
+// by lds_swz, so no exchange meets a bank conflict).
 // Schedules per length come from a variant table the autotuner picks from.
 #include <hip/hip_runtime.h>
 
@@ -24,7 +25,7 @@ namespace pacg {
 
 constexpr int BT = 1024;              // threads per workgroup
 constexpr int BPTS = 1 << 14;         // points per item
-constexpr int BLD = BPTS + BPTS / 8;  // padded LDS length per array (one slot per 8 points)
+constexpr int BLD = BPTS;             // LDS length per array (swizzled slots, no padding)
 
 // Phase twiddle of phase PH from the two LDS tables of w_{2^14}: the index
 // e = idx << (14 - log2 P) of a phase whose stages span at most 128 points
@@ -57,7 +58,6 @@ __device__ __forceinline__ void batch_item(const pacg_batch_args_t& a, const pac
                                            float* lim, const float2* twh, const float2* twl_) {
   constexpr int P = SH::P, E = SH::E, TPL = SH::TPL, NP = SH::NP;
   static_assert(BT % TPL == 0, "threads per convolution (BT / TPL convolutions per item)");
-  constexpr int LDP = P + P / 8;
   // Recompute every per-thread index inside the persistent loop: hoisted out
   // of it, the positions and LDS addresses of all phases would stay live.
   int tid = threadIdx.x;
@@ -67,7 +67,11 @@ __device__ __forceinline__ void batch_item(const pacg_batch_args_t& a, const pac
   const bool valid = g < (int)it.count;
   const uint32_t base = valid ? a.conv_off[it.first + g] : 0u;
   // phase twiddles: w_P^idx = w_{2^14}^e from two LDS tables (batch_phase_twiddle)
-  auto li = [&](int s) PACG_INL { return g * LDP + s + (s >> 3); };
+  // LDS slot of (c, r) in phase PH: convolution g at g * P, swizzled (lds_swz)
+  auto li = [&](auto PH_, int c, int r) PACG_INL {
+    constexpr int PH = decltype(PH_)::value;
+    return ((g * P) ^ lds_swz(pos<SH, PH>(tl, 0, 0))) ^ lds_swz(pos<SH, PH>(0, c, r));
+  };
 
   // Spectrum A^T k: in registers, except at 16 points per thread where it is
   // parked in this workgroup's slab of a.spec_slab (each thread re-reads only
@@ -101,7 +105,7 @@ __device__ __forceinline__ void batch_item(const pacg_batch_args_t& a, const pac
       constexpr int c = decltype(c_)::value;
       sfor<RF>([&](auto r_) PACG_INL {
         constexpr int r = decltype(r_)::value;
-        const int l = li(pos<SH, FROM>(tl, c, r));
+        const int l = li(FROM_, c, r);
         lre[l] = vr[c * RF + r];
         lim[l] = vi[c * RF + r];
       });
@@ -111,7 +115,7 @@ __device__ __forceinline__ void batch_item(const pacg_batch_args_t& a, const pac
       constexpr int c = decltype(c_)::value;
       sfor<RT>([&](auto r_) PACG_INL {
         constexpr int r = decltype(r_)::value;
-        const int l = li(pos<SH, TO>(tl, c, r));
+        const int l = li(TO_, c, r);
         vr[c * RT + r] = lre[l];
         vi[c * RT + r] = lim[l];
       });
@@ -204,7 +208,6 @@ __device__ __forceinline__ void batch_item_vt(const pacg_batch_args_t& a, const 
   asm volatile("" : "+v"(tid));
   const bool valid = it.count > 0;
   const uint32_t base = valid ? a.conv_off[it.first] : 0u;
-  auto li = [&](int s) PACG_INL { return s + (s >> 3); };
   float2* slab = reinterpret_cast<float2*>(a.spec_slab) + (size_t)blockIdx.x * BPTS;
   const float scale = 1.0f / (float)P;
 
@@ -226,8 +229,9 @@ __device__ __forceinline__ void batch_item_vt(const pacg_batch_args_t& a, const 
           vr[c * R + r] = valid ? gr[base + sp] : 0.f;
           vi[c * R + r] = valid ? gi[base + sp] : 0.f;
         } else {
-          vr[c * R + r] = lre[li(sp)];
-          vi[c * R + r] = lim[li(sp)];
+          const int l = lds_swz(pos<SH, PH>(vtl, 0, 0)) ^ lds_swz(pos<SH, PH>(0, c, r));
+          vr[c * R + r] = lre[l];
+          vi[c * R + r] = lim[l];
         }
       });
     });
@@ -267,8 +271,9 @@ __device__ __forceinline__ void batch_item_vt(const pacg_batch_args_t& a, const 
             a.yi[base + sp] = vi[c * R + r];
           }
         } else {
-          lre[li(sp)] = vr[c * R + r];
-          lim[li(sp)] = vi[c * R + r];
+          const int l = lds_swz(pos<SH, PH>(vtl, 0, 0)) ^ lds_swz(pos<SH, PH>(0, c, r));
+          lre[l] = vr[c * R + r];
+          lim[l] = vi[c * R + r];
         }
       });
     });

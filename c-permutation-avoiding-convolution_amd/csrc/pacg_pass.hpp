@@ -347,6 +347,19 @@ __device__ __forceinline__ int pos(int tl, int c, int r) {
   return (beta << (lgl + lgR)) | (r << lgl) | mu;
 }
 
+// LDS slot of point s in an exchange tile without padding: bits 0-4 of s are
+// XOR-ed with h(bits 5-7), h linear over GF(2) with h(1) = 31, h(2) = 21,
+// h(4) = 25.  For every phase layout of radix-2/4/8 schedules (pos above) the
+// 32 lanes of a half-wave then reach 32 different banks (32 banks of 4 bytes,
+// a wave64 access served as two half-waves).  Only bits 0-4 change, so the map
+// is a bijection on every aligned block of 32 points; it is linear, so the slot
+// of pos(tl, c, r) = pos(tl, 0, 0) | pos(0, c, r) (disjoint bits) is the XOR of
+// a per-thread part and a compile-time constant.
+__device__ __forceinline__ int lds_swz(int s) {
+  const int h = (-((s >> 5) & 1) & 31) ^ (-((s >> 6) & 1) & 21) ^ (-((s >> 7) & 1) & 25);
+  return s ^ h;
+}
+
 // mixed digit reversal of a point index s over all stages of the pass
 template <class SH>
 __device__ __forceinline__ int rev_pass(int s) {
