@@ -414,6 +414,21 @@ struct Vec4<double> {
   static __device__ __forceinline__ double2 join(const double* o) { return make_double2(o[0], o[1]); }
 };
 
+// Row bits of a strided FP32 tile with 16 lines (one bit) or 8 lines (two
+// bits) that are XOR-ed into the row of point s, so that the 2 or 4 rows a
+// half-wave touches in any radix-2/4/8 phase layout fall in different banks:
+// 16 lines, row bit 0 ^= parity(s bits 2-4); 8 lines, row bit 0 ^= parity(s
+// bits 2, 4) and row bit 1 ^= parity(s bits 3, 5).  Linear over GF(2).
+template <int LINES>
+__device__ __forceinline__ int row_flip(int s) {
+  if constexpr (LINES == 16)
+    return __builtin_parity((unsigned)s & 0x1Cu);
+  else
+    return __builtin_parity((unsigned)s & 0x14u) | (__builtin_parity((unsigned)s & 0x28u) << 1);
+}
+
+enum { LM_PAD = 0, LM_ROWS = 1, LM_ROWSWZ = 2, LM_LINESWZ = 3 };
+
 // ---------------------------------------------------------------------------
 // the pass kernel
 // ---------------------------------------------------------------------------
@@ -423,11 +438,25 @@ __global__ void __launch_bounds__(LINES* SH::TPL)
   static_assert(SH::valid(), "invalid pass shape");
   constexpr int P = SH::P, E = SH::E, TPL = SH::TPL, NP = SH::NP;
   constexpr int NT = LINES * TPL;
-  // LDS: strided tiles are [P][LINES+1], contiguous tiles [LINES][P + P/8 + 1]
-  // with one pad slot per 8 points (s + s/8 keeps the phase layouts of radix-4/8
-  // schedules at most 2-way bank conflicted)
-  constexpr int LDJ = LINES + 1;
-  constexpr int LDP = P + P / 8 + 1;
+  // LDS exchange tiles.  FP32 tiles are laid out so that no exchange of a
+  // radix-2/4/8 phase layout meets a bank conflict (32 banks of 4 bytes, a
+  // wave64 access served as two half-waves of 32 lanes):
+  //   LM_ROWS    strided, 32 lines or more: [P][LINES], a half-wave reads one row;
+  //   LM_ROWSWZ  strided, 8 or 16 lines: [P][LINES], the low log2(32 / LINES)
+  //              bits of the row XOR-ed with parities of the point index (row_flip);
+  //   LM_LINESWZ contiguous, 256 points or more at most 8 per thread, or 2048
+  //              points: [LINES][P] with lds_swz inside each line.
+  // Other tiles and FP64 keep LM_PAD: strided [P][LINES+1], contiguous
+  // [LINES][P + P/8 + 1] with one pad slot per 8 points.
+  constexpr bool F32 = sizeof(T) == 4;
+  constexpr int LMODE = !F32                        ? LM_PAD
+                        : LAYOUT == PACG_L_STRIDED ? (LINES >= 32                  ? LM_ROWS
+                                                      : (LINES == 16 || LINES == 8) ? LM_ROWSWZ
+                                                                                    : LM_PAD)
+                        : ((E <= 8 && P >= 256) || P >= 2048) ? LM_LINESWZ
+                                                              : LM_PAD;
+  constexpr int LDJ = (LMODE == LM_ROWS || LMODE == LM_ROWSWZ) ? LINES : LINES + 1;
+  constexpr int LDP = LMODE == LM_LINESWZ ? P : P + P / 8 + 1;
   constexpr int LDS_N = (LAYOUT == PACG_L_STRIDED) ? P * LDJ : LINES * LDP;
   constexpr bool NEED_X = NP > 1;
   __shared__ T xr[NEED_X ? LDS_N : 1];
@@ -497,11 +526,27 @@ __global__ void __launch_bounds__(LINES* SH::TPL)
   T* __restrict__ outr = reinterpret_cast<T*>(a.out_re);
   T* __restrict__ outi = reinterpret_cast<T*>(a.out_im);
 
-  auto lds_index = [&](int s) PACG_INL -> int {
-    if constexpr (LAYOUT == PACG_L_STRIDED)
-      return s * LDJ + ll;
-    else
-      return ll * LDP + s + (s >> 3);
+  // LDS slot of element (c, r) of this thread in phase PH.  Every layout is
+  // linear over GF(2) in the point index, and pos(tl, c, r) = pos(tl, 0, 0) |
+  // pos(0, c, r) with disjoint bits, so the slot is a per-thread part (the same
+  // for all elements of a phase) combined with a compile-time constant.
+  auto lds_slot = [&](auto PH_, auto C_, auto R_) PACG_INL -> int {
+    constexpr int PH = decltype(PH_)::value;
+    const int st = pos<SH, PH>(tl, 0, 0);
+    const int sc = pos<SH, PH>(0, decltype(C_)::value, decltype(R_)::value);
+    if constexpr (LMODE == LM_ROWSWZ) {
+      constexpr int NB = LINES == 16 ? 1 : 2, LGL = LINES == 16 ? 4 : 3;
+      const int pat = (sc ^ row_flip<LINES>(sc)) & ((1 << NB) - 1);
+      const int rest = (sc >> NB) << (NB + LGL);
+      return ((((st ^ row_flip<LINES>(st)) ^ pat) << LGL) | ll) + rest;
+    } else if constexpr (LMODE == LM_LINESWZ) {
+      return ((ll * P) | lds_swz(st)) ^ lds_swz(sc);
+    } else if constexpr (LAYOUT == PACG_L_STRIDED) {
+      return (st | sc) * LDJ + ll;
+    } else {
+      const int sp = st | sc;
+      return ll * LDP + sp + (sp >> 3);
+    }
   };
 
   // Phase 0 of a contiguous line holds R consecutive points per group:
@@ -596,7 +641,7 @@ __global__ void __launch_bounds__(LINES* SH::TPL)
       constexpr int c = decltype(c_)::value;
       sfor<RF>([&](auto r_) PACG_INL {
         constexpr int r = decltype(r_)::value;
-        const int li = lds_index(pos<SH, FROM>(tl, c, r));
+        const int li = lds_slot(FROM_, c_, r_);
         xr[li] = vr[c * RF + r];
         xi[li] = vi[c * RF + r];
       });
@@ -606,7 +651,7 @@ __global__ void __launch_bounds__(LINES* SH::TPL)
       constexpr int c = decltype(c_)::value;
       sfor<RT>([&](auto r_) PACG_INL {
         constexpr int r = decltype(r_)::value;
-        const int li = lds_index(pos<SH, TO>(tl, c, r));
+        const int li = lds_slot(TO_, c_, r_);
         vr[c * RT + r] = xr[li];
         vi[c * RT + r] = xi[li];
       });
