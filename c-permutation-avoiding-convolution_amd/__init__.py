@@ -90,6 +90,8 @@ def _load():
     L.pacg_batch_replay.argtypes = [_vp]
     L.pacg_batch_sync.argtypes = [_vp]
     L.pacg_plan_autotune.argtypes = [_vp, _vp, _vp, _vp, _vp, C.c_int]
+    L.pacg_plan_launch_candidates.argtypes = [_vp, C.c_int, C.POINTER(C.c_char_p), C.c_int]
+    L.pacg_plan_pin_launch.argtypes = [_vp, C.c_int, C.c_char_p]
     L.pacg_bw_probe.argtypes = [C.c_int, _vp, _vp, C.c_uint64, _vp]
     L.pacg_dropin_release.argtypes = []
     L.pacg_dropin_release.restype = None
@@ -324,6 +326,19 @@ class Plan:
         if rc < 0:
             raise PacgError("pacg_plan_autotune failed: %s" % last_error())
         return rc
+
+    def candidates(self, launch: int):
+        """Names of the compiled kernels (tile shapes) that can run launch `launch` of convolve."""
+        names = (C.c_char_p * 64)()
+        n = lib.pacg_plan_launch_candidates(self.ptr, int(launch), names, 64)
+        if n < 0:
+            raise PacgError("pacg_plan_launch_candidates failed: %s" % last_error())
+        return [names[i].decode() for i in range(min(n, 64))]
+
+    def pin(self, launch: int, name: str):
+        """Run launch `launch` of convolve with the named kernel (one of candidates(launch))."""
+        if lib.pacg_plan_pin_launch(self.ptr, int(launch), name.encode()):
+            raise PacgError("pacg_plan_pin_launch failed: %s" % last_error())
 
     def profile(self, x: DeviceComplex, y: DeviceComplex, reps: int = 10):
         """Mean duration (ms) of every kernel launch of one convolve, HIP events on the plan stream."""

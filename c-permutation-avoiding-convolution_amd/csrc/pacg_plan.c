@@ -732,6 +732,66 @@ static int run_conv(pacg_plan* pl, int prec, const void* xr, const void* xi, voi
  * shape of its pass (same stages, kind, layout and precisions) on the given
  * buffers, out of place x -> y, and keep the fastest.  Returns the number of
  * launches whose kernel changed, or minus an error code. */
+/* The kernel slot of launch i of a convolution and its current kernel (NULL
+ * for a pair launch or an index out of range).  Any compiled kernel of the
+ * same pass (same_pass: precision, kind, layout, stages, spectrum precision
+ * and engine; any lines per workgroup) can take the slot. */
+static const pacg_variant_t* launch_slot(pacg_plan* pl, int i, const pacg_variant_t*** slot) {
+  step_t st;
+  int kind;
+  if (i < 0 || i >= conv_launches(pl)) return NULL;
+  conv_launch(pl, i, &st, &kind);
+  if (st.pair >= 0) return NULL;
+  int prec = pidx(pl), vsp = kind == PACG_K_FUSED ? pl->d.spectrum_precision : prec;
+  *slot = &pl->ax[st.axis].pass[st.pass].v[prec][kind][vsp];
+  return **slot;
+}
+
+static int same_pass(const pacg_variant_t* a, const pacg_variant_t* b) {
+  return a->prec == b->prec && a->kind == b->kind && a->layout == b->layout && a->nstages == b->nstages &&
+         a->stage_code == b->stage_code && a->spec_prec == b->spec_prec && a->engine == b->engine;
+}
+
+int pacg_plan_launch_candidates(pacg_plan* pl, int launch, const char** names, int max_names) {
+  if (!pl || max_names < 0 || (max_names > 0 && !names))
+    return -pacg_fail(PACG_ERR_ARG, "pacg_plan_launch_candidates: bad argument");
+  const pacg_variant_t** slot = NULL;
+  const pacg_variant_t* cur = launch_slot(pl, launch, &slot);
+  if (!cur) return -pacg_fail(PACG_ERR_ARG, "launch %d has no pass kernel to choose", launch);
+  int n = 0;
+  for (int t = 0; t <= pacg_num_variant_tables; ++t) {
+    const pacg_variant_t* tab = t < pacg_num_variant_tables ? pacg_variant_tables[t] : pacg_mfma_variants;
+    const int cnt = t < pacg_num_variant_tables ? pacg_variant_counts[t] : pacg_num_mfma_variants;
+    for (int j = 0; j < cnt; ++j)
+      if (same_pass(&tab[j], cur)) {
+        if (n < max_names) names[n] = tab[j].name;
+        ++n;
+      }
+  }
+  return n;
+}
+
+int pacg_plan_pin_launch(pacg_plan* pl, int launch, const char* name) {
+  if (!pl || !name) return pacg_fail(PACG_ERR_ARG, "pacg_plan_pin_launch: bad argument");
+  const pacg_variant_t** slot = NULL;
+  const pacg_variant_t* cur = launch_slot(pl, launch, &slot);
+  if (!cur) return pacg_fail(PACG_ERR_ARG, "launch %d has no pass kernel to choose", launch);
+  for (int t = 0; t <= pacg_num_variant_tables; ++t) {
+    const pacg_variant_t* tab = t < pacg_num_variant_tables ? pacg_variant_tables[t] : pacg_mfma_variants;
+    const int cnt = t < pacg_num_variant_tables ? pacg_variant_counts[t] : pacg_num_mfma_variants;
+    for (int j = 0; j < cnt; ++j)
+      if (same_pass(&tab[j], cur) && strcmp(tab[j].name, name) == 0) {
+        *slot = &tab[j];
+        if (pl->graph) {
+          pacg_graph_destroy(pl->graph);
+          pl->graph = NULL;
+        }
+        return PACG_OK;
+      }
+  }
+  return pacg_fail(PACG_ERR_ARG, "'%s' is not a tile shape of launch %d (current: %s)", name, launch, cur->name);
+}
+
 int pacg_plan_autotune(pacg_plan* pl, const void* xr, const void* xi, void* yr, void* yi, int reps) {
   if (!pl || !xr || !xi || !yr || !yi || reps < 1) return -pacg_fail(PACG_ERR_ARG, "pacg_plan_autotune: bad argument");
   if (!pl->has_spec) return -pacg_fail(PACG_ERR_STATE, "no kernel spectrum set");
@@ -757,9 +817,7 @@ int pacg_plan_autotune(pacg_plan* pl, const void* xr, const void* xi, void* yr, 
       const int cnt = t < pacg_num_variant_tables ? pacg_variant_counts[t] : pacg_num_mfma_variants;
       for (int j = 0; j < cnt && !rc; ++j) {
         const pacg_variant_t* v = &tab[j];
-        if (v->prec != cur->prec || v->kind != cur->kind || v->layout != cur->layout || v->nstages != cur->nstages ||
-            v->stage_code != cur->stage_code || v->spec_prec != cur->spec_prec || v->engine != cur->engine)
-          continue;
+        if (!same_pass(v, cur)) continue;
         ps->v[prec][kind][vsp] = v;
         const void* s_re = kind == PACG_K_FUSED ? pl->spec_re : NULL;
         const void* s_im = kind == PACG_K_FUSED ? pl->spec_im : NULL;

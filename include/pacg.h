@@ -116,6 +116,14 @@ int pacg_graph_replay(pacg_plan* plan);
  * error code.  Drops a captured graph (capture again afterwards). */
 int pacg_plan_autotune(pacg_plan* plan, const void* x_re, const void* x_im, void* y_re, void* y_im, int reps);
 
+/* Tile shapes of launch i (0-based, in launch order) of pacg_convolve: writes
+ * up to max_names kernel names (static strings) of the compiled kernels that
+ * can run it and returns how many there are, or minus an error code (a pair
+ * launch has one kernel and no choice).  pacg_plan_pin_launch makes the named
+ * one current, as the autotuner would; it drops a captured graph. */
+int pacg_plan_launch_candidates(pacg_plan* plan, int launch, const char** names, int max_names);
+int pacg_plan_pin_launch(pacg_plan* plan, int launch, const char* name);
+
 /* Timing aid: runs `reps` convolutions with a HIP event before every kernel
  * launch (on the plan stream) and writes the mean duration of launch i to
  * launch_ms[i] and its kernel name to names[i] (if names != NULL).  Returns

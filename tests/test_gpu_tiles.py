@@ -1,0 +1,70 @@
+"""Every compiled tile shape of every launch gives the same convolution.
+
+The autotuner may pick any kernel of a pass (lines per workgroup, hence LDS
+layout: padded, unpadded rows, row-parity or lds_swz swizzles).  Each launch
+is pinned to each of its candidates in turn; the output must match the
+default tiles to FP32 rounding, and the default tiles match the oracle
+(FP32 <= 2e-6, FP64 <= 1e-12 relative L2)."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+CASES = [
+    # contiguous [8,8,4] (lds_swz), strided [4,8,8] (rows / row-parity), fused strided [8,8]
+    ((64, 256, 256), [[8, 8], [4, 8, 8], [8, 8, 4]], 1),
+    # the 256-point fused strided tile of the 256^3 plan
+    ((256, 64, 256), [[4, 8, 8], [8, 8], [8, 8, 4]], 1),
+    # 2048-point contiguous lines (16 points per thread), radix-4 strided pass
+    ((1024, 2048), [[4, 4, 4, 4, 4], [4, 4, 4, 4, 4, 2]], 1),
+    # FP64 tiles keep the padded layouts
+    ((64, 64, 256), [[8, 8], [8, 8], [4, 8, 8]], 0),
+]
+
+
+def rnd(seed, shape):
+    g = np.random.default_rng(seed)
+    return g.uniform(-1, 1, shape) + 1j * g.uniform(-1, 1, shape)
+
+
+@pytest.mark.parametrize("dims,scheds,prec", CASES)
+def test_every_tile_shape_of_every_launch(gpu, oracle, dims, scheds, prec):
+    x = rnd(31, dims)
+    k = rnd(32, dims)
+    y_ref, _ = oracle.pa_convolution(x, k, scheds)
+    dt = np.float32 if prec == 1 else np.float64
+    tol_ref = 2e-6 if prec == 1 else 1e-12
+    tol_tile = 1e-6 if prec == 1 else 1e-14
+    pl = gpu.Plan(dims, schedules=scheds, precision=prec, spectrum_precision=prec)
+    pl.set_kernel(k)
+    dx = gpu.DeviceComplex.from_numpy(x, dt)
+    dy = pl.alloc()
+    pl.convolve(dx, dy)
+    y0 = dy.numpy()
+    assert rel_l2(y0, y_ref) < tol_ref
+    default = [nm for nm, _ in pl.profile(dx, dy, 1)]
+    tried = 0
+    for i, name0 in enumerate(default):
+        cands = pl.candidates(i)
+        assert name0 in cands, (i, name0, cands)
+        for nm in cands:
+            pl.pin(i, nm)
+            dy.upload(np.zeros(dims, complex))
+            pl.convolve(dx, dy)
+            assert rel_l2(dy.numpy(), y0) < tol_tile, (i, nm)
+            tried += 1
+        pl.pin(i, name0)
+    assert tried > len(default)
+    assert [nm for nm, _ in pl.profile(dx, dy, 1)] == default
+
+
+def test_pin_rejects_foreign_kernels(gpu):
+    pl = gpu.Plan((64, 256), schedules=[[8, 8], [8, 8, 4]], precision=gpu.PREC_F32)
+    pl.set_kernel(rnd(33, (64, 256)))
+    with pytest.raises(gpu.PacgError, match="not a tile shape"):
+        pl.pin(0, "f64_dif_c_8x8x4_h_l8")
+    with pytest.raises(gpu.PacgError, match="no pass kernel"):
+        pl.candidates(99)
