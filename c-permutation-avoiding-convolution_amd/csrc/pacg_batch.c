@@ -61,9 +61,13 @@ int pacg_batch_num_variants(int log2n) {
   return c;
 }
 
+/* convolutions per item: the 1024 threads' worth of TPL-thread groups, or for
+ * lines of 8192 points and more (the kernel's virtual-thread path) as many
+ * lines as fill 2^14 points */
 static int per_item(int m, int v) {
   const pacg_batch_variant_t* r = variant_row(m, v);
   if (!r) return 1;
+  if ((1 << m) >= 8 * BATCH_THREADS) return (1 << m) < SLAB_POINTS ? SLAB_POINTS >> m : 1;
   int g = BATCH_THREADS / r->tpl;
   return g < 1 ? 1 : g;
 }

@@ -194,20 +194,28 @@ __device__ __forceinline__ void batch_item(const pacg_batch_args_t& a, const pac
   __syncthreads();  // LDS is reused by the next item
 }
 
-// Lines of BT * 8 points and more (2^13, 2^14): each thread runs VT = TPL / BT
-// virtual threads of 8 points in turn, the line stays in LDS between phases (the same
-// LDS traffic as the exchange model, half the live registers) and the
-// spectrum is parked in the workgroup's slab.  One convolution per item.
+// Lines of BT * 8 points and more (2^13, 2^14): an item holds G = 2^14 / P
+// convolutions (2 at 2^13, 1 at 2^14) and each thread runs VT = G * TPL / BT
+// virtual threads of 8 points in turn (virtual thread vtl: convolution
+// vtl / TPL, thread vtl % TPL of it).  The lines stay in LDS between phases
+// (the same LDS traffic as the exchange model, half the live registers) and
+// the spectra are parked in the workgroup's slab.
 template <class SH>
 __device__ __forceinline__ void batch_item_vt(const pacg_batch_args_t& a, const pacg_batch_item_t it, float* lre,
                                               float* lim, const float2* twh, const float2* twl_) {
-  constexpr int P = SH::P, E = SH::E, NP = SH::NP;
-  constexpr int VT = SH::TPL / BT;
-  static_assert(VT * BT == SH::TPL && E <= 8, "virtual threads");
+  constexpr int P = SH::P, E = SH::E, NP = SH::NP, TPL = SH::TPL;
+  constexpr int G = BPTS / P;
+  constexpr int VT = G * TPL / BT;
+  static_assert(VT * BT == G * TPL && TPL % BT == 0 && E <= 8 && (G == 1 || G == 2), "virtual threads");
   int tid = threadIdx.x;
   asm volatile("" : "+v"(tid));
-  const bool valid = it.count > 0;
-  const uint32_t base = valid ? a.conv_off[it.first] : 0u;
+  bool valid_g[G];
+  uint32_t base_g[G];
+#pragma unroll
+  for (int g = 0; g < G; ++g) {
+    valid_g[g] = g < (int)it.count;
+    base_g[g] = valid_g[g] ? a.conv_off[it.first + g] : 0u;
+  }
   float2* slab = reinterpret_cast<float2*>(a.spec_slab) + (size_t)blockIdx.x * BPTS;
   const float scale = 1.0f / (float)P;
 
@@ -219,17 +227,23 @@ __device__ __forceinline__ void batch_item_vt(const pacg_batch_args_t& a, const 
     constexpr bool from_global = decltype(FROMG_)::value;
     constexpr int tail = decltype(TAIL_)::value;
     constexpr int R = 1 << SH::lgR(PH);
+    // TPL >= BT: the convolution index is uniform over the workgroup
+    const int g = G == 1 ? 0 : vtl / TPL;
+    const int tl = G == 1 ? vtl : vtl % TPL;
+    const bool valid = G == 1 ? valid_g[0] : (g ? valid_g[G - 1] : valid_g[0]);
+    const uint32_t base = G == 1 ? base_g[0] : (g ? base_g[G - 1] : base_g[0]);
+    const int lg = g * P;  // this convolution's LDS and slab region
     float vr[E], vi[E];
     sfor<E / R>([&](auto c_) PACG_INL {
       constexpr int c = decltype(c_)::value;
       sfor<R>([&](auto r_) PACG_INL {
         constexpr int r = decltype(r_)::value;
-        const int sp = pos<SH, PH>(vtl, c, r);
+        const int sp = pos<SH, PH>(tl, c, r);
         if constexpr (from_global) {
           vr[c * R + r] = valid ? gr[base + sp] : 0.f;
           vi[c * R + r] = valid ? gi[base + sp] : 0.f;
         } else {
-          const int l = lds_swz(pos<SH, PH>(vtl, 0, 0)) ^ lds_swz(pos<SH, PH>(0, c, r));
+          const int l = lg + (lds_swz(pos<SH, PH>(tl, 0, 0)) ^ lds_swz(pos<SH, PH>(0, c, r)));
           vr[c * R + r] = lre[l];
           vi[c * R + r] = lim[l];
         }
@@ -237,9 +251,9 @@ __device__ __forceinline__ void batch_item_vt(const pacg_batch_args_t& a, const 
     });
     if constexpr (MODE == M_DIF) {
       phase_stages<float, SH, PH, M_DIF>(vr, vi);
-      batch_phase_twiddle<SH, PH, false, false>(vr, vi, vtl, twh, twl_);
+      batch_phase_twiddle<SH, PH, false, false>(vr, vi, tl, twh, twl_);
     } else {
-      batch_phase_twiddle<SH, PH, true, false>(vr, vi, vtl, twh, twl_);
+      batch_phase_twiddle<SH, PH, true, false>(vr, vi, tl, twh, twl_);
       phase_stages<float, SH, PH, M_DITC>(vr, vi);
     }
     // tail: 0 write LDS, 1 k spectrum -> slab, 2 x: product + conj(A) phase 0 -> LDS, 3 -> y
@@ -248,7 +262,7 @@ __device__ __forceinline__ void batch_item_vt(const pacg_batch_args_t& a, const 
       if constexpr (tail == 2) {
         sfor<R>([&](auto r_) PACG_INL {
           constexpr int r = decltype(r_)::value;
-          const float2 h = slab[pos<SH, PH>(vtl, c, r)];
+          const float2 h = slab[lg + pos<SH, PH>(tl, c, r)];
           cmul<float>(vr[c * R + r], vi[c * R + r], h.x, h.y);
           vr[c * R + r] *= scale;
           vi[c * R + r] *= scale;
@@ -262,16 +276,16 @@ __device__ __forceinline__ void batch_item_vt(const pacg_batch_args_t& a, const 
       constexpr int c = decltype(c_)::value;
       sfor<R>([&](auto r_) PACG_INL {
         constexpr int r = decltype(r_)::value;
-        const int sp = pos<SH, PH>(vtl, c, r);
+        const int sp = pos<SH, PH>(tl, c, r);
         if constexpr (tail == 1) {
-          slab[sp] = make_float2(vr[c * R + r], vi[c * R + r]);
+          slab[lg + sp] = make_float2(vr[c * R + r], vi[c * R + r]);
         } else if constexpr (tail == 3) {
           if (valid) {
             a.yr[base + sp] = vr[c * R + r];
             a.yi[base + sp] = vi[c * R + r];
           }
         } else {
-          const int l = lds_swz(pos<SH, PH>(vtl, 0, 0)) ^ lds_swz(pos<SH, PH>(0, c, r));
+          const int l = lg + (lds_swz(pos<SH, PH>(tl, 0, 0)) ^ lds_swz(pos<SH, PH>(0, c, r)));
           lre[l] = vr[c * R + r];
           lim[l] = vi[c * R + r];
         }

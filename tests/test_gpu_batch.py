@@ -102,3 +102,17 @@ def test_batch_axis_plan_convolves_rows(gpu, oracle, prec, tol):
         pl.execute(op, dx)
         ref = np.fft.fft(x, axis=1) if op == gpu.OP_FFT else np.fft.ifft(x, axis=1)
         assert rel_l2(dx.numpy(), ref) < tol
+
+
+def test_ragged_items_of_8192_points(gpu, oracle):
+    """Items of 8192-point convolutions hold two; an odd count leaves a last item with one."""
+    ms = np.array([13, 12, 13, 14, 13, 8], dtype=np.int32)
+    xs = [rnd(300 + i, 1 << int(m)) for i, m in enumerate(ms)]
+    ks = [rnd(400 + i, 1 << int(m)) for i, m in enumerate(ms)]
+    b = gpu.Batch(ms)
+    dx = gpu.DeviceComplex.from_numpy(b.pack(xs), np.float32)
+    dk = gpu.DeviceComplex.from_numpy(b.pack(ks), np.float32)
+    dy = b.alloc()
+    b.convolve(dx, dk, dy)
+    b.sync()
+    check(b, xs, ks, dy, oracle, {m: gpu.Batch.schedule(m, b.variant(m)) for m in range(8, 15)})
