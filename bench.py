@@ -136,6 +136,60 @@ def cpu_baseline(x, k, reps):
             "n": int(n)}
 
 
+def _cpu_throughput_worker(seed, barrier, out_q):
+    """One single-threaded process of the CPU throughput figure: the reference's
+    permutation_avoiding_convolution_3d at 256^3, radix 4, a warm-up call, then one call
+    timed from a barrier shared by all processes.  Inputs are its own random arrays
+    (the arithmetic does not depend on the values)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from oracle import RefLib  # checker / baseline only
+
+    g = np.random.default_rng(seed)
+    shape = (N1 * N1 * N1,)
+    re, im, hre0, him0 = (g.standard_normal(shape) for _ in range(4))
+    ref = RefLib()
+    _dp = C.POINTER(C.c_double)
+    tws = ref._tws([N1] * 3, 4)
+    flat = [a for pair in tws for a in pair]
+    f = ref.L.permutation_avoiding_convolution_3d
+    dt = 0.0
+    for it in range(2):
+        xr, xi, hr, hi = re.copy(), im.copy(), hre0.copy(), him0.copy()
+        if it == 1:
+            barrier.wait()
+        t0 = time.perf_counter()
+        f(xr.ctypes.data_as(_dp), xi.ctypes.data_as(_dp), N1, N1, N1, 4, *flat, hr.ctypes.data_as(_dp),
+          hi.ctypes.data_as(_dp))
+        dt = time.perf_counter() - t0
+    ref._free(tws, [N1] * 3, 4)
+    out_q.put(dt)
+
+
+def cpu_throughput(procs):
+    """`procs` independent single-threaded processes of the reference convolution at once (child
+    processes started fresh, none touches the GPU): aggregate conv/s = procs / slowest call."""
+    import multiprocessing as mp
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from oracle import RefLib  # checker / baseline only
+
+    if not RefLib.available():
+        return None
+    ctx = mp.get_context("spawn")
+    barrier = ctx.Barrier(procs)
+    q = ctx.Queue()
+    ps = [ctx.Process(target=_cpu_throughput_worker, args=(100 + i, barrier, q)) for i in range(procs)]
+    for p_ in ps:
+        p_.start()
+    dts = [q.get(timeout=600) for _ in ps]
+    for p_ in ps:
+        p_.join(timeout=60)
+    slow = max(dts)
+    return {"value": procs / slow, "unit": "conv/s", "processes": procs, "kind": "reference",
+            "sample": "256^3 complex FP64 PA convolution, radix 4, %d single-threaded processes at once, one timed "
+                      "call each after a warm-up (slowest %.3f s, fastest %.3f s)" % (procs, slow, min(dts))}
+
+
 def bench_fp64_configs(pacg, synth, steps, warmup):
     """Configs 1 (1D 2^16, radix 2) and 2 (2D 2048^2, radix 4 with one radix-2 stage), FP64:
     device-resident plan rate, and the drop-in entry point's rate from host arrays (PCIe included)."""
@@ -271,6 +325,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--cpu-reps", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-procs", type=int, default=16,
+                    help="processes of the CPU throughput figure (0: skip); the box's CPU share is 16 per GPU")
     ap.add_argument("--spectrum", choices=["f32", "f64"], default="f32",
                     help="storage of the FP64-computed spectrum on the headline FP32 path")
     ap.add_argument("--no-tune", action="store_true", help="skip the tile autotuner")
@@ -470,6 +526,9 @@ def main():
     if not a.no_cpu:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(x, k, a.cpu_reps)
+        if a.cpu_procs > 0:
+            log("cpu throughput, %d processes ..." % a.cpu_procs)
+            out["cpu_baseline_throughput"] = cpu_throughput(min(a.cpu_procs, os.cpu_count() or 1))
     print(json.dumps(out), flush=True)
 
 
