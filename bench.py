@@ -357,6 +357,7 @@ def main():
         dx = pacg.DeviceComplex.from_numpy(b.pack(xs), np.float32)
         dk = pacg.DeviceComplex.from_numpy(b.pack(ks), np.float32)
         dy = b.alloc()
+        b.autotune(dx, dk, dy, 3)  # the schedules the bench line reports
         b.capture(dx, dk, dy)
         for _ in range(a.steps):
             b.replay()
