@@ -190,13 +190,13 @@ def cpu_throughput(procs):
                       "call each after a warm-up (slowest %.3f s, fastest %.3f s)" % (procs, slow, min(dts))}
 
 
-def bench_fp64_configs(pacg, synth, steps, warmup):
-    """Configs 1 (1D 2^16, radix 2) and 2 (2D 2048^2, radix 4 with one radix-2 stage), FP64:
-    device-resident plan rate, and the drop-in entry point's rate from host arrays (PCIe included)."""
+def bench_fp64_workloads(pacg, synth, steps, warmup):
+    """1D 2^16 (radix 2) and 2D 2048^2 (radix 4 with one radix-2 stage), FP64: device-resident
+    plan rate, and the drop-in entry point's rate from host arrays (PCIe included)."""
     out = {}
-    for cfg, sched_of, name in ((1, lambda n: [2] * 16, "c1_1d_65536_r2"),
-                                (2, lambda n: [4, 4, 4, 4, 4, 2], "c2_2d_2048x2048_r4")):
-        x, k = synth.config_inputs(cfg)
+    for wl, sched_of, name in (("1d_65536", lambda n: [2] * 16, "fp64_1d_65536_r2"),
+                               ("2d_2048x2048", lambda n: [4, 4, 4, 4, 4, 2], "fp64_2d_2048x2048_r4")):
+        x, k = synth.workload_inputs(wl)
         n = x.size
         d = x.ndim
         scheds = [sched_of(s) for s in x.shape]
@@ -237,7 +237,7 @@ def bench_fp64_configs(pacg, synth, steps, warmup):
 def bench_mfma(pacg, synth, steps, warmup):
     """4096 x 256 complex, radix 8-8-8-8 / 8-8-4, radix-8 stages on split-FP16 MFMA (seed 3),
     beside the same plan on the VALU butterflies."""
-    x, k = synth.config_inputs(4)
+    x, k = synth.workload_inputs("2d_4096x256")
     n = x.size
     scheds = [[8, 8, 8, 8], [8, 8, 4]]
     ref = np.fft.ifft2(np.fft.fft2(x) * np.fft.fft2(k))
@@ -366,7 +366,7 @@ def main():
         return
 
     if a.profile_mfma:
-        x4, k4 = synth.config_inputs(4)
+        x4, k4 = synth.workload_inputs("2d_4096x256")
         pl = pacg.Plan(x4.shape, schedules=[[8, 8, 8, 8], [8, 8, 4]], precision=pacg.PREC_F32,
                        spectrum_precision=pacg.PREC_F32, mfma=True)
         pl.set_kernel(k4)
@@ -381,7 +381,7 @@ def main():
         return
 
     t0 = time.time()
-    x, k = synth.config_inputs(3)
+    x, k = synth.workload_inputs("3d_256cubed")
     n = x.size
     log("inputs ready (%.1f s)" % (time.time() - t0))
     scheds = SCHEDS32
@@ -515,8 +515,8 @@ def main():
         "launches_fp32": [{"kernel": nm, "ms": t} for nm, t in prof32],
         "launches_fp64": [{"kernel": nm, "ms": t} for nm, t in prof64],
     }
-    log("fp64 configs 1 and 2 ...")
-    out.update(bench_fp64_configs(pacg, synth, a.steps, a.warmup))
+    log("fp64 1D 2^16 and 2D 2048^2 ...")
+    out.update(bench_fp64_workloads(pacg, synth, a.steps, a.warmup))
     if not a.no_mfma:
         log("mfma workload ...")
         out["mfma_4096x256"] = bench_mfma(pacg, synth, a.steps, a.warmup)

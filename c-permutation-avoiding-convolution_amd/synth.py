@@ -68,19 +68,19 @@ def gaussian_stencil_3d(n: int, sigma: float = 3.0) -> np.ndarray:
 
 
 # ---- the five workloads ------------------------------------------------------
-def config_inputs(cfg: int):
-    """(x, k) complex128 arrays for workload 1..5 (5 returns lists of pairs)."""
-    if cfg == 1:  # 1D 2^16, U(-1,1), seed 0
+def workload_inputs(name: str):
+    """(x, k) complex128 arrays of a named workload ("batched_4096" returns lists of pairs)."""
+    if name == "1d_65536":  # 1D 2^16, U(-1,1), seed 0
         return complex_uniform(0, (1 << 16,), 0), complex_uniform(0, (1 << 16,), 1)
-    if cfg == 2:  # 2D 2048^2, U(-1,1), seed 1
+    if name == "2d_2048x2048":  # 2D 2048^2, U(-1,1), seed 1
         return complex_uniform(1, (2048, 2048), 0), complex_uniform(1, (2048, 2048), 1)
-    if cfg == 3:  # 3D 256^3, x ~ N(0,1), Gaussian stencil sigma = 3, seed 2
+    if name == "3d_256cubed":  # 3D 256^3, x ~ N(0,1), Gaussian stencil sigma = 3, seed 2
         return complex_normal(2, (256, 256, 256), 0), gaussian_stencil_3d(256, 3.0)
-    if cfg == 4:  # 2D 4096 x 256, U(-1,1), seed 3
+    if name == "2d_4096x256":  # 2D 4096 x 256, U(-1,1), seed 3
         return complex_uniform(3, (4096, 256), 0), complex_uniform(3, (4096, 256), 1)
-    if cfg == 5:
+    if name == "batched_4096":  # 4096 pairs of lengths 2^U{8..14}, U(-1,1), seed 4
         return batched_inputs(4, 4096)
-    raise ValueError(cfg)
+    raise ValueError(name)
 
 
 def batched_lengths(seed: int, count: int, lo: int = 8, hi: int = 14) -> np.ndarray:

@@ -76,7 +76,7 @@ def test_mfma_block_scaling_handles_wide_range(gpu, oracle):
 
 def test_anisotropic_4096x256_full_size(gpu, synth):
     """Config 4: 4096 x 256, U(-1,1), seed 3, radix 8-8-8-8 and 8-8-4, MFMA radix-8 stages."""
-    x, k = synth.config_inputs(4)
+    x, k = synth.workload_inputs("2d_4096x256")
     pl = gpu.Plan(x.shape, schedules=[[8, 8, 8, 8], [8, 8, 4]], precision=gpu.PREC_F32,
                   spectrum_precision=gpu.PREC_F32, mfma=True)
     pl.set_kernel(k)

@@ -1,8 +1,8 @@
 """GPU parity: the HIP pass kernels against the oracle on the same inputs.
 
-FP64 bar: relative L2 <= 1e-12 (north_star's FP64 tolerance); the digit
+FP64 bar: relative L2 <= 1e-12 against the oracle's FP64 convolution; the digit
 reversal / spectrum order is compared exactly (integer bookkeeping).
-FP32 bar: relative L2 <= 2e-6 against FP64 (north_star's FP32 tolerance).
+FP32 bar: relative L2 <= 2e-6 against FP64 (FP32 butterflies, FP64-generated twiddles).
 """
 from __future__ import annotations
 

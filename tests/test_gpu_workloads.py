@@ -1,4 +1,4 @@
-"""The benchmark configurations at their full sizes.
+"""The benchmark workloads at their full sizes.
 
 Parity at these sizes: against the oracle where it finishes in seconds, else
 against numpy.fft in complex128 (the oracle matches numpy to 1e-15 at the
@@ -12,9 +12,9 @@ from conftest import rel_l2
 pytestmark = pytest.mark.gpu
 
 
-def test_config1_dropin_1d_65536_r2(gpu, oracle, synth):
+def test_dropin_1d_65536_r2(gpu, oracle, synth):
     """1D n = 2^16, radix 2, through the reference-named entry point (host arrays)."""
-    x, k = synth.config_inputs(1)
+    x, k = synth.workload_inputs("1d_65536")
     n = x.size
     y_ref, ath_ref = oracle.pa_convolution(x, k, [[2] * 16])
     re, im = np.ascontiguousarray(x.real), np.ascontiguousarray(x.imag)
@@ -24,9 +24,9 @@ def test_config1_dropin_1d_65536_r2(gpu, oracle, synth):
     assert rel_l2(hr + 1j * hi, ath_ref) < 1e-12  # h <- A^T h, unscaled, bit-reversed order
 
 
-def test_config2_2048x2048_radix4_fp64(gpu, oracle, synth):
+def test_2d_2048x2048_radix4_fp64(gpu, oracle, synth):
     """2D 2048^2 with radix-4 stages and one radix-2 on top (2048 = 4^5 * 2), FP64."""
-    x, k = synth.config_inputs(2)
+    x, k = synth.workload_inputs("2d_2048x2048")
     sched = [4, 4, 4, 4, 4, 2]
     pl = gpu.Plan(x.shape, schedules=[sched, sched], precision=gpu.PREC_F64)
     pl.set_kernel(k)
@@ -46,9 +46,9 @@ def test_config2_2048x2048_radix4_fp64(gpu, oracle, synth):
     assert rel_l2(d2.numpy(), y2_ref) < 1e-12
 
 
-def test_config2_dropin_r2_matches_reference_algorithm(gpu, oracle, synth):
+def test_2d_2048x2048_dropin_r2_matches_reference_algorithm(gpu, oracle, synth):
     """The reference can only run 2048^2 with radix 2 (supports_radix_4(11) is false)."""
-    x, k = synth.config_inputs(2)
+    x, k = synth.workload_inputs("2d_2048x2048")
     x, k = x[:256], k[:256]  # 256 x 2048 keeps the oracle fast
     y_ref, ath_ref = oracle.pa_convolution(x, k, [[2] * 8, [2] * 11])
     re, im = np.ascontiguousarray(x.real.ravel()), np.ascontiguousarray(x.imag.ravel())
@@ -64,10 +64,10 @@ def test_config2_dropin_r2_matches_reference_algorithm(gpu, oracle, synth):
     (1, 2e-6, [[8, 8, 4]] * 3, True, 3),
     (1, 2e-6, [[8, 8, 4], [4, 8, 8], [8, 8, 4]], True, 3),
 ])
-def test_config3_256cubed(gpu, synth, prec, tol, scheds, pairs, launches):
+def test_3d_256cubed(gpu, synth, prec, tol, scheds, pairs, launches):
     """3D 256^3, radix-8 schedules, Gaussian stencil (sum 1): FP64 and FP32 against numpy,
     plus the delta-kernel identity at full size; one axis per launch and pair launches."""
-    x, k = synth.config_inputs(3)
+    x, k = synth.workload_inputs("3d_256cubed")
     pl = gpu.Plan(x.shape, schedules=scheds, precision=prec, spectrum_precision=prec, pairs=pairs)
     assert pl.launches_per_convolve() == launches, pl.describe()
     pl.set_kernel(k)
