@@ -67,6 +67,20 @@ def time_convs(pacg, pl, dx, dy, steps, warmup):
     return ms / steps
 
 
+def step_distribution(pacg, pl, dx, dy, steps):
+    """Each of `steps` convolutions timed on its own (HIP events around it, a host sync between):
+    median, standard deviation, min and max in ms."""
+    t = pacg.Timer(pl.stream)
+    ms = []
+    for _ in range(steps):
+        t.start()
+        pl.convolve(dx, dy)
+        ms.append(t.stop())
+    v = np.array(ms)
+    return {"median_ms": float(np.median(v)), "std_ms": float(v.std()), "min_ms": float(v.min()),
+            "max_ms": float(v.max()), "steps": steps}
+
+
 def profile_launches(pacg, pl, dx, dy, reps):
     return pl.profile(dx, dy, reps)
 
@@ -409,6 +423,7 @@ def main():
         return
     ms32 = time_convs(pacg, pl32, dx32, dy32, a.steps, a.warmup)
     log("fp32: %.3f ms/conv  plan: %s" % (ms32, pl32.describe()))
+    dist32 = step_distribution(pacg, pl32, dx32, dy32, a.steps)
     prof32 = profile_launches(pacg, pl32, dx32, dy32, max(5, a.steps // 2))
     y32 = dy32.numpy()
     nl32 = len(prof32)
@@ -481,6 +496,7 @@ def main():
         "steps": a.steps,
         "warmup": a.warmup,
         "ms_per_step": ms32,
+        "step_distribution": dist32,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
