@@ -75,7 +75,7 @@ __device__ __forceinline__ void batch_item(const pacg_batch_args_t& a, const pac
 
   // Spectrum A^T k: in registers, except at 16 points per thread where it is
   // parked in this workgroup's slab of a.spec_slab (each thread re-reads only
-  // what it wrote, so no barrier is involved; the slab stays in L2).
+  // what it wrote, so no barrier is involved).
   constexpr bool SPEC_REG = E <= 8;
   float vr[E], vi[E];   // working points
   float pr[(E <= 8 && P <= 4096) ? E : 1], pi[(E <= 8 && P <= 4096) ? E : 1];  // x prefetch
@@ -198,8 +198,10 @@ __device__ __forceinline__ void batch_item(const pacg_batch_args_t& a, const pac
 // convolutions (2 at 2^13, 1 at 2^14) and each thread runs VT = G * TPL / BT
 // virtual threads of 8 points in turn (virtual thread vtl: convolution
 // vtl / TPL, thread vtl % TPL of it).  The lines stay in LDS between phases
-// (the same LDS traffic as the exchange model, half the live registers) and
-// the spectra are parked in the workgroup's slab.
+// (the same LDS traffic as the exchange model, half the live registers).  The
+// spectrum A^T k leaves the last forward phase of k in the registers of the
+// thread that later holds the same positions of x (phase 0 of virtual thread
+// v, for both), so it stays there: 2 x VT x 8 values, no global round trip.
 template <class SH>
 __device__ __forceinline__ void batch_item_vt(const pacg_batch_args_t& a, const pacg_batch_item_t it, float* lre,
                                               float* lim, const float2* twh, const float2* twl_) {
@@ -216,12 +218,14 @@ __device__ __forceinline__ void batch_item_vt(const pacg_batch_args_t& a, const 
     valid_g[g] = g < (int)it.count;
     base_g[g] = valid_g[g] ? a.conv_off[it.first + g] : 0u;
   }
-  float2* slab = reinterpret_cast<float2*>(a.spec_slab) + (size_t)blockIdx.x * BPTS;
   const float scale = 1.0f / (float)P;
+  float sr[VT][E], si[VT][E];  // the spectrum at the phase-0 positions of each virtual thread
 
-  // one phase for one virtual thread: read (global or LDS), compute, write back
-  auto run = [&](auto PH_, auto MODE_, auto FROMG_, auto TAIL_, int vtl, const float* __restrict__ gr,
+  // one phase for one virtual thread: read (global or LDS), compute, write back;
+  // V = the virtual thread index when the phase touches the spectrum (tails 1, 2)
+  auto run = [&](auto PH_, auto MODE_, auto FROMG_, auto TAIL_, auto V_, int vtl, const float* __restrict__ gr,
                  const float* __restrict__ gi) PACG_INL {
+    constexpr int V = decltype(V_)::value;
     constexpr int PH = decltype(PH_)::value;
     constexpr int MODE = decltype(MODE_)::value;
     constexpr bool from_global = decltype(FROMG_)::value;
@@ -232,7 +236,7 @@ __device__ __forceinline__ void batch_item_vt(const pacg_batch_args_t& a, const 
     const int tl = G == 1 ? vtl : vtl % TPL;
     const bool valid = G == 1 ? valid_g[0] : (g ? valid_g[G - 1] : valid_g[0]);
     const uint32_t base = G == 1 ? base_g[0] : (g ? base_g[G - 1] : base_g[0]);
-    const int lg = g * P;  // this convolution's LDS and slab region
+    const int lg = g * P;  // this convolution's LDS region
     float vr[E], vi[E];
     sfor<E / R>([&](auto c_) PACG_INL {
       constexpr int c = decltype(c_)::value;
@@ -256,14 +260,13 @@ __device__ __forceinline__ void batch_item_vt(const pacg_batch_args_t& a, const 
       batch_phase_twiddle<SH, PH, true, false>(vr, vi, tl, twh, twl_);
       phase_stages<float, SH, PH, M_DITC>(vr, vi);
     }
-    // tail: 0 write LDS, 1 k spectrum -> slab, 2 x: product + conj(A) phase 0 -> LDS, 3 -> y
+    // tail: 0 write LDS, 1 k spectrum -> sr/si, 2 x: product + conj(A) phase 0 -> LDS, 3 -> y
     sfor<E / R>([&](auto c_) PACG_INL {
       constexpr int c = decltype(c_)::value;
       if constexpr (tail == 2) {
         sfor<R>([&](auto r_) PACG_INL {
           constexpr int r = decltype(r_)::value;
-          const float2 h = slab[lg + pos<SH, PH>(tl, c, r)];
-          cmul<float>(vr[c * R + r], vi[c * R + r], h.x, h.y);
+          cmul<float>(vr[c * R + r], vi[c * R + r], sr[V][c * R + r], si[V][c * R + r]);
           vr[c * R + r] *= scale;
           vi[c * R + r] *= scale;
         });
@@ -278,7 +281,8 @@ __device__ __forceinline__ void batch_item_vt(const pacg_batch_args_t& a, const 
         constexpr int r = decltype(r_)::value;
         const int sp = pos<SH, PH>(tl, c, r);
         if constexpr (tail == 1) {
-          slab[lg + sp] = make_float2(vr[c * R + r], vi[c * R + r]);
+          sr[V][c * R + r] = vr[c * R + r];
+          si[V][c * R + r] = vi[c * R + r];
         } else if constexpr (tail == 3) {
           if (valid) {
             a.yr[base + sp] = vr[c * R + r];
@@ -293,8 +297,9 @@ __device__ __forceinline__ void batch_item_vt(const pacg_batch_args_t& a, const 
     });
   };
 
-  // A^T k (spectrum to the slab), then A^T x, product, conj(A).  The virtual
-  // threads run one after the other (no unrolling: one set of points live).
+  // A^T k (the spectrum into sr/si), then A^T x, product, conj(A).  The virtual
+  // threads run one after the other (no unrolling: one set of points live),
+  // except in phase 0, where each keeps its own spectrum registers.
   sfor<2>([&](auto pass_) PACG_INL {
     constexpr int pass = decltype(pass_)::value;
     const float* gr = pass == 0 ? a.kr : a.xr;
@@ -303,10 +308,20 @@ __device__ __forceinline__ void batch_item_vt(const pacg_batch_args_t& a, const 
       constexpr int PH = NP - 1 - decltype(i_)::value;
       constexpr int tail = PH > 0 ? 0 : (pass == 0 ? 1 : 2);
       __syncthreads();
+      if constexpr (tail == 0) {
 #pragma nounroll
-      for (int v = 0; v < VT; ++v)
-        run(std::integral_constant<int, PH>{}, std::integral_constant<int, M_DIF>{},
-            std::integral_constant<bool, PH == NP - 1>{}, std::integral_constant<int, tail>{}, tid + v * BT, gr, gi);
+        for (int v = 0; v < VT; ++v)
+          run(std::integral_constant<int, PH>{}, std::integral_constant<int, M_DIF>{},
+              std::integral_constant<bool, PH == NP - 1>{}, std::integral_constant<int, tail>{},
+              std::integral_constant<int, 0>{}, tid + v * BT, gr, gi);
+      } else {
+        sfor<VT>([&](auto v_) PACG_INL {
+          constexpr int v = decltype(v_)::value;
+          run(std::integral_constant<int, PH>{}, std::integral_constant<int, M_DIF>{},
+              std::integral_constant<bool, PH == NP - 1>{}, std::integral_constant<int, tail>{},
+              std::integral_constant<int, v>{}, tid + v * BT, gr, gi);
+        });
+      }
     });
   });
   sfor<NP - 1>([&](auto i_) PACG_INL {
@@ -316,7 +331,7 @@ __device__ __forceinline__ void batch_item_vt(const pacg_batch_args_t& a, const 
 #pragma nounroll
     for (int v = 0; v < VT; ++v)
       run(std::integral_constant<int, PH>{}, std::integral_constant<int, M_DITC>{}, std::integral_constant<bool, false>{},
-          std::integral_constant<int, tail>{}, tid + v * BT, a.xr, a.xi);
+          std::integral_constant<int, tail>{}, std::integral_constant<int, 0>{}, tid + v * BT, a.xr, a.xi);
   });
   __syncthreads();  // LDS is reused by the next item
 }
