@@ -27,28 +27,51 @@ constexpr int BT = 1024;              // threads per workgroup
 constexpr int BPTS = 1 << 14;         // points per item
 constexpr int BLD = BPTS;             // LDS length per array (swizzled slots, no padding)
 
-// Phase twiddle of phase PH from the two LDS tables of w_{2^14}: the index
-// e = idx << (14 - log2 P) of a phase whose stages span at most 128 points
-// (lglam(top + 1) <= 7) is a multiple of 128, so w = hi[e >> 7] exactly and
-// the second table and the product are skipped (SINGLE; the virtual-thread
-// path keeps the product, where the shorter code spilled registers).
-template <class SH, int PH, bool CJ, bool SINGLE = true>
+// Phase twiddle of phase PH.  The R points of a butterfly group take
+// w^(mu * kr), kr = 0 .. R-1, with w = w_P^(2^sh); only the base b = w^mu
+// comes from the two LDS tables of w_{2^14} (index e = (mu << sh) << (14 -
+// log2 P); when the phase's stages span at most 128 points, e is a multiple
+// of 128 and b = hi[e >> 7] exactly).  The other powers follow as products
+// b^k = b^(k/2) * b^(k - k/2), at most three products deep: 6 complex
+// multiplies instead of 7 table reads and products for a radix-8 group.
+template <class SH, int PH, bool CJ>
 __device__ __forceinline__ void batch_phase_twiddle(float* vr, float* vi, int tl, const float2* twh,
                                                     const float2* twl_) {
-  constexpr int TW_SHIFT = 14 - SH::LGP;
-  if constexpr (SINGLE && SH::lglam(SH::phi(PH) + 1) <= 7) {
-    phase_twiddle_f<float, SH, PH, CJ>(vr, vi, tl, [&](int idx, float& wr, float& wi) PACG_INL {
-      const float2 h = twh[(idx << TW_SHIFT) >> 7];
-      wr = h.x;
-      wi = h.y;
-    });
-  } else {
-    phase_twiddle_f<float, SH, PH, CJ>(vr, vi, tl, [&](int idx, float& wr, float& wi) PACG_INL {
-      const int e = idx << TW_SHIFT;
-      const float2 h = twh[e >> 7];
-      const float2 l = twl_[e & 127];
-      wr = h.x * l.x - h.y * l.y;
-      wi = h.x * l.y + h.y * l.x;
+  constexpr int lo = SH::plo(PH);
+  if constexpr (lo > 0) {
+    constexpr int TW_SHIFT = 14 - SH::LGP;
+    constexpr int R = 1 << SH::lgR(PH);
+    constexpr int lgl = SH::lglam(lo);
+    constexpr int sh = SH::LGP - SH::lglam(SH::phi(PH) + 1);
+    constexpr bool ONE_TABLE = SH::lglam(SH::phi(PH) + 1) <= 7;
+    static_assert(R <= 8, "phase radix");
+    sfor<SH::E / R>([&](auto c_) PACG_INL {
+      constexpr int c = decltype(c_)::value;
+      const int mu = (tl + SH::TPL * c) & ((1 << lgl) - 1);
+      const int e = (mu << sh) << TW_SHIFT;
+      float pr[R], pi[R];
+      if constexpr (ONE_TABLE) {
+        const float2 h = twh[e >> 7];
+        pr[1] = h.x;
+        pi[1] = h.y;
+      } else {
+        const float2 h = twh[e >> 7];
+        const float2 l = twl_[e & 127];
+        pr[1] = h.x * l.x - h.y * l.y;
+        pi[1] = h.x * l.y + h.y * l.x;
+      }
+      if constexpr (CJ) pi[1] = -pi[1];
+      sfor<R - 2>([&](auto k_) PACG_INL {
+        constexpr int k = 2 + decltype(k_)::value;
+        constexpr int a = k / 2, b = k - k / 2;
+        pr[k] = pr[a] * pr[b] - pi[a] * pi[b];
+        pi[k] = pr[a] * pi[b] + pi[a] * pr[b];
+      });
+      sfor<R>([&](auto r_) PACG_INL {
+        constexpr int r = decltype(r_)::value;
+        constexpr int kr = SH::rev_phase(PH, r);
+        if constexpr (kr != 0) cmul<float>(vr[c * R + r], vi[c * R + r], pr[kr], pi[kr]);
+      });
     });
   }
 }
@@ -255,9 +278,9 @@ __device__ __forceinline__ void batch_item_vt(const pacg_batch_args_t& a, const 
     });
     if constexpr (MODE == M_DIF) {
       phase_stages<float, SH, PH, M_DIF>(vr, vi);
-      batch_phase_twiddle<SH, PH, false, false>(vr, vi, tl, twh, twl_);
+      batch_phase_twiddle<SH, PH, false>(vr, vi, tl, twh, twl_);
     } else {
-      batch_phase_twiddle<SH, PH, true, false>(vr, vi, tl, twh, twl_);
+      batch_phase_twiddle<SH, PH, true>(vr, vi, tl, twh, twl_);
       phase_stages<float, SH, PH, M_DITC>(vr, vi);
     }
     // tail: 0 write LDS, 1 k spectrum -> sr/si, 2 x: product + conj(A) phase 0 -> LDS, 3 -> y
