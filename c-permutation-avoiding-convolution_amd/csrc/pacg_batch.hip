@@ -320,9 +320,12 @@ __device__ __forceinline__ void batch_item_vt(const pacg_batch_args_t& a, const 
     });
   };
 
-  // A^T k (the spectrum into sr/si), then A^T x, product, conj(A).  The virtual
-  // threads run one after the other (no unrolling: one set of points live),
-  // except in phase 0, where each keeps its own spectrum registers.
+  // A^T k (the spectrum into sr/si), then A^T x, product, conj(A).  In the
+  // forward phases the virtual threads run one after the other (one set of
+  // points live; unrolled, they spill), except in phase 0, where each keeps its
+  // own spectrum registers.  The conj(A) phases, with the spectrum gone, are
+  // unrolled so that the LDS reads of one virtual thread overlap the other's
+  // arithmetic.
   sfor<2>([&](auto pass_) PACG_INL {
     constexpr int pass = decltype(pass_)::value;
     const float* gr = pass == 0 ? a.kr : a.xr;
@@ -351,7 +354,7 @@ __device__ __forceinline__ void batch_item_vt(const pacg_batch_args_t& a, const 
     constexpr int PH = 1 + decltype(i_)::value;
     constexpr int tail = PH == NP - 1 ? 3 : 0;
     __syncthreads();
-#pragma nounroll
+#pragma unroll
     for (int v = 0; v < VT; ++v)
       run(std::integral_constant<int, PH>{}, std::integral_constant<int, M_DITC>{}, std::integral_constant<bool, false>{},
           std::integral_constant<int, tail>{}, std::integral_constant<int, 0>{}, tid + v * BT, a.xr, a.xi);
