@@ -20,7 +20,7 @@
 #define LMIN 8
 #define LMAX 14
 #define BATCH_THREADS 1024
-#define SLAB_POINTS (1 << 14)
+#define ITEM_POINTS (1 << 14)
 
 struct pacg_batch {
   int count;
@@ -39,7 +39,6 @@ struct pacg_batch {
   int n_items;
   int cap_items;
   void* d_tw;
-  void* d_slab;
 };
 
 static const pacg_batch_variant_t* variant_row(int m, int v) {
@@ -67,7 +66,7 @@ int pacg_batch_num_variants(int log2n) {
 static int per_item(int m, int v) {
   const pacg_batch_variant_t* r = variant_row(m, v);
   if (!r) return 1;
-  if ((1 << m) >= 8 * BATCH_THREADS) return (1 << m) < SLAB_POINTS ? SLAB_POINTS >> m : 1;
+  if ((1 << m) >= 8 * BATCH_THREADS) return (1 << m) < ITEM_POINTS ? ITEM_POINTS >> m : 1;
   int g = BATCH_THREADS / r->tpl;
   return g < 1 ? 1 : g;
 }
@@ -118,7 +117,6 @@ int pacg_batch_destroy(pacg_batch* b) {
   pacg_dev_free(b->d_conv_off);
   pacg_dev_free(b->d_items);
   pacg_dev_free(b->d_tw);
-  pacg_dev_free(b->d_slab);
   if (b->own_stream && b->stream) pacg_stream_destroy(b->stream);
   free(b->log2n);
   free(b->offset);
@@ -172,8 +170,7 @@ int pacg_batch_create(pacg_batch** out, int count, const int* log2_len) {
   b->cap_items = count + 1;
   if (pacg_dev_malloc((void**)&b->d_conv_off, (size_t)count * sizeof(uint32_t)) ||
       pacg_dev_malloc((void**)&b->d_items, (size_t)b->cap_items * sizeof(pacg_batch_item_t)) ||
-      pacg_dev_malloc(&b->d_tw, (size_t)SLAB_POINTS * 2 * sizeof(float)) ||
-      pacg_dev_malloc(&b->d_slab, (size_t)cus * SLAB_POINTS * 2 * sizeof(float))) {
+      pacg_dev_malloc(&b->d_tw, (size_t)ITEM_POINTS * 2 * sizeof(float))) {
     rc = pacg_fail(PACG_ERR_NOMEM, "device allocation failed");
     goto fail;
   }
@@ -193,7 +190,7 @@ int pacg_batch_create(pacg_batch** out, int count, const int* log2_len) {
     }
   }
   /* w_{2^14} in FP64 on the device, rounded to FP32 (twiddle_factors.c:17-131 values) */
-  if (pacg_gen_twiddles(b->d_tw, PACG_F32, SLAB_POINTS, SLAB_POINTS, 1, b->stream)) {
+  if (pacg_gen_twiddles(b->d_tw, PACG_F32, ITEM_POINTS, ITEM_POINTS, 1, b->stream)) {
     rc = pacg_fail(PACG_ERR_HIP, "twiddle generation failed");
     goto fail;
   }
@@ -241,7 +238,6 @@ static int launch(pacg_batch* b, const float* xr, const float* xi, const float* 
   a.items = b->d_items;
   a.n_items = (uint32_t)b->n_items;
   a.tw = b->d_tw;
-  a.spec_slab = b->d_slab;
   int blocks = b->n_items < b->blocks ? b->n_items : b->blocks;
   int rc = pacg_batch_launch(&a, blocks, b->stream);
   if (rc) return pacg_fail(PACG_ERR_HIP, "batched launch failed: %s", pacg_hip_error_string(rc));

@@ -29,7 +29,6 @@ typedef struct {
   const pacg_batch_item_t* items;
   uint32_t n_items;
   const void* tw;                    /* w_{2^14}^p, p < 2^14, (re, im) float pairs */
-  void* spec_slab;                   /* 2^14 float pairs per workgroup (spectra of 2^14-point items) */
 } pacg_batch_args_t;
 
 typedef struct {

@@ -96,14 +96,12 @@ __device__ __forceinline__ void batch_item(const pacg_batch_args_t& a, const pac
     return ((g * P) ^ lds_swz(pos<SH, PH>(tl, 0, 0))) ^ lds_swz(pos<SH, PH>(0, c, r));
   };
 
-  // Spectrum A^T k: in registers, except at 16 points per thread where it is
-  // parked in this workgroup's slab of a.spec_slab (each thread re-reads only
-  // what it wrote, so no barrier is involved).
-  constexpr bool SPEC_REG = E <= 8;
-  float vr[E], vi[E];   // working points
-  float pr[(E <= 8 && P <= 4096) ? E : 1], pi[(E <= 8 && P <= 4096) ? E : 1];  // x prefetch
-  float sr[SPEC_REG ? E : 1], si[SPEC_REG ? E : 1];
-  float2* slab = reinterpret_cast<float2*>(a.spec_slab) + (size_t)blockIdx.x * BPTS;
+  // Spectrum A^T k and the read-ahead of x: in registers (at most 8 points per
+  // thread below 2^13 points, so both fit beside the working points).
+  static_assert(E <= 8 && P <= 4096, "register spectrum and x read-ahead");
+  float vr[E], vi[E];  // working points
+  float pr[E], pi[E];  // x, loaded while k is transformed
+  float sr[E], si[E];  // the spectrum
 
   auto load_top = [&](const float* __restrict__ br, const float* __restrict__ bi, float* dr, float* di) PACG_INL {
     constexpr int PH = NP - 1;
@@ -153,41 +151,21 @@ __device__ __forceinline__ void batch_item(const pacg_batch_args_t& a, const pac
     });
   };
 
-  // Up to 2^12 points with 8 per thread, x is loaded while k is transformed;
-  // beyond that the registers hold the spectrum and x is loaded afterwards.
-  constexpr bool PF = E <= 8 && P <= 4096;
   load_top(a.kr, a.ki, vr, vi);
-  if constexpr (PF) load_top(a.xr, a.xi, pr, pi);
-  dif();  // A^T k
-  if constexpr (SPEC_REG) {
+  load_top(a.xr, a.xi, pr, pi);  // x arrives while k is transformed
+  dif();                         // A^T k
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-      sr[e] = vr[e];
-      si[e] = vi[e];
-      if constexpr (PF) {
-        vr[e] = pr[e];
-        vi[e] = pi[e];
-      }
-    }
-  } else {
-#pragma unroll
-    for (int e = 0; e < E; ++e) slab[e * BT + tid] = make_float2(vr[e], vi[e]);
+  for (int e = 0; e < E; ++e) {
+    sr[e] = vr[e];
+    si[e] = vi[e];
+    vr[e] = pr[e];
+    vi[e] = pi[e];
   }
-  if constexpr (!PF) load_top(a.xr, a.xi, vr, vi);
   dif();  // A^T x
   const float scale = 1.0f / (float)P;
 #pragma unroll
   for (int e = 0; e < E; ++e) {
-    float hr, hi;
-    if constexpr (SPEC_REG) {
-      hr = sr[e];
-      hi = si[e];
-    } else {
-      const float2 h = slab[e * BT + tid];
-      hr = h.x;
-      hi = h.y;
-    }
-    cmul<float>(vr[e], vi[e], hr, hi);
+    cmul<float>(vr[e], vi[e], sr[e], si[e]);
     vr[e] *= scale;
     vi[e] *= scale;
   }
