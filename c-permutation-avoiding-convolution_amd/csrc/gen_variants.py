@@ -27,12 +27,12 @@ LONG_SHAPES = [[4, 4, 4, 4, 4, 2], [2] * 11, [8, 8, 8, 4], [4, 8, 8, 8]]
 E_TARGET = 16
 
 
-def phases(shape):
-    """Greedy bottom-up grouping of stages into register phases (product <= 16)."""
+def phases(shape, et=E_TARGET):
+    """Greedy bottom-up grouping of stages into register phases (product <= et)."""
     out = []
     cur = 1
     for g, r in enumerate(shape):
-        if cur > 1 and cur * r > E_TARGET:
+        if cur > 1 and cur * r > et:
             out.append(g - 1)
             cur = 1
         cur *= r
@@ -40,23 +40,23 @@ def phases(shape):
     return out
 
 
-def codes(shape):
+def codes(shape, et=E_TARGET):
     lg = {2: 1, 4: 2, 8: 3}
     sc = 0
     for g, r in enumerate(shape):
         sc |= lg[r] << (2 * g)
-    ph = phases(shape)
+    ph = phases(shape, et)
     pc = 0
     for p, top in enumerate(ph):
         pc |= top << (4 * p)
     return sc, len(ph), pc
 
 
-def geometry(shape, prec):
+def geometry(shape, prec, et=E_TARGET):
     P = 1
     for r in shape:
         P *= r
-    ph = phases(shape)
+    ph = phases(shape, et)
     E = 1
     lo = 0
     for top in ph:
@@ -80,9 +80,9 @@ def geometry(shape, prec):
     return P, E, tpl, strided_lines, nl
 
 
-def tile_options(shape, prec, layout):
+def tile_options(shape, prec, layout, et=E_TARGET):
     """Lines per workgroup to compile: the default first, then the autotuner's alternatives."""
-    P, E, tpl, sl, nl = geometry(shape, prec)
+    P, E, tpl, sl, nl = geometry(shape, prec, et)
     elem = 4 if prec == "float" else 8
     if layout == "PACG_L_STRIDED":
         opts = [sl]
@@ -109,15 +109,21 @@ def emit(prec: str, outdir: str):
     pconst = "PACG_F64" if prec == "double" else "PACG_F32"
     spec_types = ["double"] if prec == "double" else ["float", "double"]
     rows = []
-    for shape in SHAPES + LONG_SHAPES:
-        sc, npph, pc = codes(shape)
-        P, E, tpl, sl, nl = geometry(shape, prec)
+    # FP64 long lines default to 8-point phases: more phases, but half the
+    # registers per thread (7 instead of 4 waves per SIMD; 2048^2 passes 37.6
+    # -> 33 us); the 16-point phases stay as autotuner alternatives
+    e8 = [s for s in LONG_SHAPES if prec == "double" and phases(s, 8) != phases(s)]
+    jobs = [(shape, E_TARGET, "", 8 if shape in e8 else 0) for shape in SHAPES + LONG_SHAPES]
+    jobs += [(shape, 8, "_e8", 0) for shape in e8]
+    for shape, et, suffix, rank0 in jobs:
+        sc, npph, pc = codes(shape, et)
+        P, E, tpl, sl, nl = geometry(shape, prec, et)
         shname = "Shape<%d, 0x%Xu, %d, 0x%Xu>" % (len(shape), sc, npph, pc)
-        sname = "x".join(str(r) for r in shape)
+        sname = "x".join(str(r) for r in shape) + suffix
         for kind in KINDS:
             for layout in (LAYOUTS if shape in SHAPES else ["PACG_L_CONTIG"]):
                 specs = spec_types if kind == "PACG_K_FUSED" else [T]
-                for rank, ln in enumerate(tile_options(shape, prec, layout)):
+                for rank, ln in enumerate(tile_options(shape, prec, layout, et), start=rank0):
                     for ts in specs:
                         fn = "launch_pass<%s, %s, %s, %s, %s, %d>" % (T, ts, shname, kind, layout, ln)
                         sp = "PACG_F64" if ts == "double" else "PACG_F32"

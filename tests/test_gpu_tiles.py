@@ -1,7 +1,8 @@
 """Every compiled tile shape of every launch gives the same convolution.
 
 The autotuner may pick any kernel of a pass (lines per workgroup, hence LDS
-layout: padded, unpadded rows, row-parity or lds_swz swizzles).  Each launch
+layout: padded, unpadded rows, row-parity or lds_swz swizzles; points per
+thread, for the FP64 2048-point lines).  Each launch
 is pinned to each of its candidates in turn; the output must match the
 default tiles to FP32 rounding, and the default tiles match the oracle
 (FP32 <= 2e-6, FP64 <= 1e-12 relative L2)."""
@@ -22,6 +23,8 @@ CASES = [
     ((1024, 2048), [[4, 4, 4, 4, 4], [4, 4, 4, 4, 4, 2]], 1),
     # FP64 tiles keep the padded layouts
     ((64, 64, 256), [[8, 8], [8, 8], [4, 8, 8]], 0),
+    # FP64 2048-point contiguous lines: split re/im exchange, 16- and 8-point phases
+    ((64, 2048), [[8, 8], [4, 4, 4, 4, 4, 2]], 0),
 ]
 
 
