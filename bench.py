@@ -166,6 +166,87 @@ def cpu_baseline(x, k, reps):
             "n": int(n)}
 
 
+def cpu_baseline_workloads(synth, gpu):
+    """The reference's own entry points on one host core for the other workloads, at the radix the
+    reference runs them with (2 for 2^16, 2048^2 and the batched lengths, 4 for 4096 x 256): one
+    warm-up and one or more timed calls each, h re-copied before every call.  `gpu` maps a workload
+    to the GPU's ms for the same work (None when it was not measured)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from oracle import RefLib  # checker / baseline only
+
+    if not RefLib.available():
+        return None
+    ref = RefLib()
+    _dp = C.POINTER(C.c_double)
+    L = ref.L
+
+    def arrays(x, k):
+        return [np.ascontiguousarray(a, dtype=np.float64) for a in (x.real, x.imag, k.real, k.imag)]
+
+    def timed(call, reps):
+        ts = []
+        for it in range(reps + 1):
+            t = call()
+            if it:
+                ts.append(t)
+        return float(np.mean(ts))
+
+    def one(f, dims, r, x, k):
+        re0, im0, hr0, hi0 = arrays(x.ravel(), k.ravel())
+        tws = ref._tws(dims, r)
+        flat = [a for pair in tws for a in pair]
+
+        def call():
+            re, im, hr, hi = re0.copy(), im0.copy(), hr0.copy(), hi0.copy()
+            t0 = time.perf_counter()
+            if len(dims) == 1:
+                f(re.ctypes.data_as(_dp), im.ctypes.data_as(_dp), dims[0], r, *flat, hr.ctypes.data_as(_dp),
+                  hi.ctypes.data_as(_dp))
+            else:
+                f(re.ctypes.data_as(_dp), im.ctypes.data_as(_dp), *dims, r, *flat, hr.ctypes.data_as(_dp),
+                  hi.ctypes.data_as(_dp))
+            return time.perf_counter() - t0
+        return call, tws
+
+    out = {}
+    for wl, f, r, reps in (("1d_65536", L.permutation_avoiding_convolution, 2, 10),
+                           ("2d_2048x2048", L.permutation_avoiding_convolution_2d, 2, 1),
+                           ("2d_4096x256", L.permutation_avoiding_convolution_2d, 4, 3)):
+        x, k = synth.workload_inputs(wl)
+        dims = list(x.shape)
+        call, tws = one(f, dims, r, x, k)
+        sec = timed(call, reps)
+        ref._free(tws, dims, r)
+        out[wl] = {"s_per_conv": sec, "radix": r, "timed_calls": reps}
+    # batched: every pair through the 1D entry point, twiddles per length prepared beforehand
+    xs, ks = synth.batched_inputs(4, 4096)
+    lens = sorted({len(x) for x in xs})
+    tw = {n: ref._tws([n], 2) for n in lens}
+    data = [arrays(x, k) for x, k in zip(xs, ks)]
+
+    def batch_call():
+        work = [[a.copy() for a in d] for d in data]
+        t0 = time.perf_counter()
+        for (re, im, hr, hi) in work:
+            n = re.size
+            flat = [a for pair in tw[n] for a in pair]
+            L.permutation_avoiding_convolution(re.ctypes.data_as(_dp), im.ctypes.data_as(_dp), n, 2, *flat,
+                                               hr.ctypes.data_as(_dp), hi.ctypes.data_as(_dp))
+        return time.perf_counter() - t0
+    sec = timed(batch_call, 1)
+    for n in lens:
+        ref._free(tw[n], [n], 2)
+    out["batched_4096"] = {"s_per_batch": sec, "radix": 2, "timed_calls": 1}
+    for wl, e in out.items():
+        g = gpu.get(wl)
+        sec = e.get("s_per_conv", e.get("s_per_batch"))
+        e["gpu_ms"] = g
+        e["gpu_speedup"] = (sec * 1e3 / g) if g else None
+    return {"cores": 1, "kind": "reference", "unit": "s per call", "workloads": out,
+            "note": "FP64 reference entry points on one host core; the GPU figures are the FP64 plans "
+                    "(1D, 2048^2), the FP32 VALU plan (4096 x 256) and the FP32 batched launch"}
+
+
 def _cpu_throughput_worker(seed, barrier, out_q):
     """One single-threaded process of the CPU throughput figure: the reference's
     permutation_avoiding_convolution_3d at 256^3, radix 4, a warm-up call, then one call
@@ -567,6 +648,12 @@ def main():
         if a.cpu_procs > 0:
             log("cpu throughput, %d processes ..." % a.cpu_procs)
             out["cpu_baseline_throughput"] = cpu_throughput(min(a.cpu_procs, os.cpu_count() or 1))
+        log("cpu baseline, other workloads ...")
+        gpu = {"1d_65536": out.get("fp64_1d_65536_r2", {}).get("ms_per_conv"),
+               "2d_2048x2048": out.get("fp64_2d_2048x2048_r4", {}).get("ms_per_conv"),
+               "2d_4096x256": out.get("mfma_4096x256", {}).get("valu", {}).get("ms_per_conv"),
+               "batched_4096": out.get("batched", {}).get("ms_per_batch")}
+        out["cpu_baseline_workloads"] = cpu_baseline_workloads(synth, gpu)
     print(json.dumps(out), flush=True)
 
 

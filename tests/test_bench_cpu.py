@@ -1,0 +1,43 @@
+"""bench.py's host-side pieces, on the CPU: the command-line refusal of several GPUs and the
+per-workload CPU baseline (the reference library built from its sources, when present)."""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "c-permutation-avoiding-convolution_amd"))
+
+
+def test_several_gpus_are_refused():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode != 0
+    assert "one GPU only" in r.stdout + r.stderr
+
+
+def test_cpu_baseline_workloads_report_every_workload():
+    import bench
+    import synth
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from oracle import RefLib
+
+    if not RefLib.available():
+        pytest.skip("oracle/_ref not built")
+    gpu = {"1d_65536": 0.02, "2d_2048x2048": 0.15, "2d_4096x256": None, "batched_4096": 0.2}
+    r = bench.cpu_baseline_workloads(synth, gpu)
+    assert r["kind"] == "reference" and r["cores"] == 1
+    w = r["workloads"]
+    assert set(w) == {"1d_65536", "2d_2048x2048", "2d_4096x256", "batched_4096"}
+    assert w["1d_65536"]["radix"] == 2 and w["2d_4096x256"]["radix"] == 4
+    for name, e in w.items():
+        sec = e.get("s_per_conv", e.get("s_per_batch"))
+        assert sec > 0.0
+        if gpu[name] is None:
+            assert e["gpu_speedup"] is None
+        else:
+            assert e["gpu_speedup"] == pytest.approx(sec * 1e3 / gpu[name])
