@@ -102,12 +102,14 @@ static int build_items(pacg_batch* b, unsigned mask) {
   return rc;
 }
 
-static int cmp_order_ctx_len;
-static const int* cmp_len;
-static int cmp_desc(const void* a, const void* b) {
-  int ia = *(const int*)a, ib = *(const int*)b;
-  if (cmp_len[ia] != cmp_len[ib]) return cmp_len[ib] - cmp_len[ia];
-  return ia - ib;
+/* order[] <- pair indices sorted by length, longest first, input order within a
+ * length (a counting sort over the few possible lengths: no shared state, so
+ * batches may be created from several threads at once) */
+static void sort_longest_first(const int* log2n, int count, int* order) {
+  int start[LMAX + 2] = {0};
+  for (int i = 0; i < count; ++i) start[LMAX - log2n[i] + 1]++;
+  for (int m = 1; m <= LMAX + 1; ++m) start[m] += start[m - 1];
+  for (int i = 0; i < count; ++i) order[start[LMAX - log2n[i]]++] = i;
 }
 
 int pacg_batch_destroy(pacg_batch* b) {
@@ -147,7 +149,6 @@ int pacg_batch_create(pacg_batch** out, int count, const int* log2_len) {
     }
     b->log2n[i] = m;
     b->offset[i] = off;
-    b->order[i] = i;
     off += 1LL << m;
   }
   b->total = off;
@@ -155,9 +156,7 @@ int pacg_batch_create(pacg_batch** out, int count, const int* log2_len) {
     pacg_batch_destroy(b);
     return pacg_fail(PACG_ERR_SIZE, "batch of %lld points exceeds 2^32", off);
   }
-  cmp_len = b->log2n;
-  (void)cmp_order_ctx_len;
-  qsort(b->order, (size_t)count, sizeof(int), cmp_desc);
+  sort_longest_first(b->log2n, count, b->order);
   for (int m = 0; m <= LMAX; ++m) b->variant[m] = 0;
   int rc = PACG_OK, cus = 0;
   if (pacg_stream_create(&b->stream)) {
