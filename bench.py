@@ -329,7 +329,7 @@ def bench_fp64_workloads(pacg, synth, steps, warmup):
         # drop-in entry point, host arrays in and out (the reference's radix: 2 for both)
         re0, im0 = np.ascontiguousarray(x.real.ravel()), np.ascontiguousarray(x.imag.ravel())
         hr0, hi0 = np.ascontiguousarray(k.real.ravel()), np.ascontiguousarray(k.imag.ravel())
-        reps = 5
+        reps = 10
         times = []
         for it in range(reps + 1):
             re, im, hr, hi = re0.copy(), im0.copy(), hr0.copy(), hi0.copy()
@@ -341,7 +341,10 @@ def bench_fp64_workloads(pacg, synth, steps, warmup):
                                                          hr, hi)
             if it:
                 times.append(time.perf_counter() - t0)
-        entry["dropin_host_arrays_ms"] = 1000.0 * float(np.mean(times))
+        # median: a call now and then takes several times longer in the runtime's pageable copies
+        entry["dropin_host_arrays_ms"] = 1000.0 * float(np.median(times))
+        entry["dropin_host_arrays_ms_mean_min_max"] = [1000.0 * float(np.mean(times)), 1000.0 * float(np.min(times)),
+                                                       1000.0 * float(np.max(times))]
         out[name] = entry
     pacg.release()
     return out

@@ -208,6 +208,8 @@ int pacg_bw_probe(int mode, void* dst, const void* src, uint64_t bytes, void* st
 // ---- runtime glue ----------------------------------------------------------
 int pacg_dev_malloc(void** p, size_t bytes) { return (int)hipMalloc(p, bytes ? bytes : 1); }
 int pacg_dev_free(void* p) { return p ? (int)hipFree(p) : 0; }
+int pacg_host_malloc(void** p, size_t bytes) { return (int)hipHostMalloc(p, bytes, hipHostMallocDefault); }
+int pacg_host_free(void* p) { return p ? (int)hipHostFree(p) : 0; }
 int pacg_memcpy_h2d(void* dst, const void* src, size_t bytes, void* stream) {
   return (int)hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, (hipStream_t)stream);
 }

@@ -172,6 +172,8 @@ int pacg_bw_probe(int mode, void* dst, const void* src, uint64_t bytes, void* st
 /* runtime glue implemented in HIP (so the C layer needs no HIP headers) */
 int pacg_dev_malloc(void** p, size_t bytes);
 int pacg_dev_free(void* p);
+int pacg_host_malloc(void** p, size_t bytes); /* pinned host memory (hipHostMalloc) */
+int pacg_host_free(void* p);
 int pacg_memcpy_h2d(void* dst, const void* src, size_t bytes, void* stream);
 int pacg_memcpy_d2h(void* dst, const void* src, size_t bytes, void* stream);
 int pacg_memcpy_d2d(void* dst, const void* src, size_t bytes, void* stream);
