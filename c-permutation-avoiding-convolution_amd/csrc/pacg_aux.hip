@@ -241,6 +241,7 @@ const char* pacg_hip_error_string(int code) { return hipGetErrorString((hipError
 int pacg_event_create(void** ev) { return (int)hipEventCreate((hipEvent_t*)ev); }
 int pacg_event_destroy(void* ev) { return (int)hipEventDestroy((hipEvent_t)ev); }
 int pacg_event_record(void* ev, void* stream) { return (int)hipEventRecord((hipEvent_t)ev, (hipStream_t)stream); }
+int pacg_event_sync(void* ev) { return (int)hipEventSynchronize((hipEvent_t)ev); }
 int pacg_event_elapsed_ms(float* ms, void* start, void* stop) {
   return (int)hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)stop);
 }

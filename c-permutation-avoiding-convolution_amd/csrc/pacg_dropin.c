@@ -18,6 +18,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <unistd.h>
 
 #include "pacg.h"
 #include "pacg_fft.h"
@@ -112,6 +113,161 @@ static slot_t* get_slot(int nd, const int* dims, int r) {
   return victim;
 }
 
+/* ---- large arrays: pinned ring, chunked and pipelined ---------------------
+ * The runtime's pageable path streams at about 30 GB/s host to device and
+ * 17 GB/s device to host on MI355X; DMA from pinned memory runs at 56 GB/s.
+ * Arrays above STAGE_MAX_BYTES therefore go through RING_SLOTS pinned chunks:
+ * the host copy of one chunk (split over a small thread pool, one host thread
+ * alone copies too slowly) overlaps the DMA of the previous ones.  The ring is
+ * shared by all slots; every drop-in call holds g_mu while it uses it. */
+#define RING_CHUNK ((size_t)8 << 20)
+#define RING_SLOTS 3
+#define POOL_MAX 8
+#define PART_MIN ((size_t)1 << 20)
+
+static char* g_ring[RING_SLOTS];
+static void* g_ring_ev[RING_SLOTS];
+static int g_ring_state; /* 0 not tried, 1 ready, -1 unavailable (pageable path) */
+
+static pthread_t g_pool[POOL_MAX - 1];
+static int g_pool_n = -1; /* worker threads, -1 before the first use */
+static pthread_mutex_t g_pool_mu = PTHREAD_MUTEX_INITIALIZER;
+static pthread_cond_t g_pool_go = PTHREAD_COND_INITIALIZER;
+static pthread_cond_t g_pool_done = PTHREAD_COND_INITIALIZER;
+static struct {
+  char* dst;
+  const char* src;
+  size_t bytes, part;
+  int parts, next, done;
+  unsigned gen;
+} g_job;
+
+/* take parts of the current job until none is left (g_pool_mu held on entry and exit) */
+static void pool_work(void) {
+  while (g_job.next < g_job.parts) {
+    const int i = g_job.next++;
+    const size_t off = (size_t)i * g_job.part;
+    const size_t len = off + g_job.part <= g_job.bytes ? g_job.part : g_job.bytes - off;
+    pthread_mutex_unlock(&g_pool_mu);
+    memcpy(g_job.dst + off, g_job.src + off, len);
+    pthread_mutex_lock(&g_pool_mu);
+    if (++g_job.done == g_job.parts) pthread_cond_broadcast(&g_pool_done);
+  }
+}
+
+static void* pool_main(void* arg) {
+  (void)arg;
+  unsigned seen = 0;
+  pthread_mutex_lock(&g_pool_mu);
+  for (;;) {
+    while (g_job.gen == seen) pthread_cond_wait(&g_pool_go, &g_pool_mu);
+    seen = g_job.gen;
+    pool_work();
+  }
+  return NULL;
+}
+
+static void pool_start(void) {
+  long cpus = sysconf(_SC_NPROCESSORS_ONLN);
+  int want = cpus > 1 ? (int)(cpus < POOL_MAX ? cpus : POOL_MAX) - 1 : 0;
+  g_pool_n = 0;
+  for (int i = 0; i < want; ++i) {
+    if (pthread_create(&g_pool[i], NULL, pool_main, NULL)) break;
+    pthread_detach(g_pool[i]);
+    g_pool_n++;
+  }
+}
+
+/* memcpy split over the pool; the calling thread takes parts too */
+static void par_memcpy(void* dst, const void* src, size_t bytes) {
+  if (g_pool_n < 0) pool_start();
+  if (g_pool_n == 0 || bytes < 2 * PART_MIN) {
+    memcpy(dst, src, bytes);
+    return;
+  }
+  int parts = g_pool_n + 1;
+  size_t part = (bytes + (size_t)parts - 1) / (size_t)parts;
+  if (part < PART_MIN) part = PART_MIN;
+  pthread_mutex_lock(&g_pool_mu);
+  g_job.dst = (char*)dst;
+  g_job.src = (const char*)src;
+  g_job.bytes = bytes;
+  g_job.part = part;
+  g_job.parts = (int)((bytes + part - 1) / part);
+  g_job.next = 0;
+  g_job.done = 0;
+  g_job.gen++;
+  pthread_cond_broadcast(&g_pool_go);
+  pool_work();
+  while (g_job.done < g_job.parts) pthread_cond_wait(&g_pool_done, &g_pool_mu);
+  pthread_mutex_unlock(&g_pool_mu);
+}
+
+static int ring_ready(void) {
+  if (g_ring_state == 0) {
+    g_ring_state = 1;
+    for (int i = 0; i < RING_SLOTS; ++i)
+      if (pacg_host_malloc((void**)&g_ring[i], RING_CHUNK) || pacg_event_create(&g_ring_ev[i])) g_ring_state = -1;
+  }
+  return g_ring_state == 1;
+}
+
+static void ring_release(void) {
+  for (int i = 0; i < RING_SLOTS; ++i) {
+    if (g_ring_ev[i]) pacg_event_sync(g_ring_ev[i]);
+    pacg_host_free(g_ring[i]);
+    if (g_ring_ev[i]) pacg_event_destroy(g_ring_ev[i]);
+    g_ring[i] = NULL;
+    g_ring_ev[i] = NULL;
+  }
+  g_ring_state = 0;
+}
+
+/* chunk c of every transfer uses ring slot c % RING_SLOTS; g_ring_used[i]
+ * says whether slot i has a DMA recorded that a new user must wait for */
+static int g_ring_used[RING_SLOTS];
+
+static int ring_h2d(void* d, const double* h, size_t bytes, void* stream) {
+  int e = 0;
+  size_t nch = (bytes + RING_CHUNK - 1) / RING_CHUNK;
+  for (size_t c = 0; c < nch && !e; ++c) {
+    const int k = (int)(c % RING_SLOTS);
+    const size_t off = c * RING_CHUNK, len = off + RING_CHUNK <= bytes ? RING_CHUNK : bytes - off;
+    if (g_ring_used[k]) e |= pacg_event_sync(g_ring_ev[k]);  /* its previous DMA has read it */
+    if (e) break;
+    par_memcpy(g_ring[k], (const char*)h + off, len);
+    e |= pacg_memcpy_h2d((char*)d + off, g_ring[k], len, stream);
+    e |= pacg_event_record(g_ring_ev[k], stream);
+    g_ring_used[k] = 1;
+  }
+  return e;
+}
+
+static int ring_d2h(double* h, const void* d, size_t bytes, void* stream) {
+  int e = 0;
+  size_t nch = (bytes + RING_CHUNK - 1) / RING_CHUNK;
+  /* keep RING_SLOTS - 1 DMAs ahead of the host copies */
+  for (size_t c = 0; c < nch + RING_SLOTS - 1 && !e; ++c) {
+    if (c < nch) {
+      const int k = (int)(c % RING_SLOTS);
+      const size_t off = c * RING_CHUNK, len = off + RING_CHUNK <= bytes ? RING_CHUNK : bytes - off;
+      /* slot k's previous chunk was copied out one iteration ago (host side), and any
+       * host-to-device DMA still reading it is ordered before this copy on the stream */
+      e |= pacg_memcpy_d2h(g_ring[k], (const char*)d + off, len, stream);
+      e |= pacg_event_record(g_ring_ev[k], stream);
+      g_ring_used[k] = 1;
+    }
+    if (c + 1 >= RING_SLOTS && c + 1 - RING_SLOTS < nch) {
+      const size_t j = c + 1 - RING_SLOTS;
+      const int k = (int)(j % RING_SLOTS);
+      const size_t off = j * RING_CHUNK, len = off + RING_CHUNK <= bytes ? RING_CHUNK : bytes - off;
+      e |= pacg_event_sync(g_ring_ev[k]);
+      if (!e) par_memcpy((char*)h + off, g_ring[k], len);
+    }
+  }
+  return e;
+}
+
 /* host array h -> device array d; k (0..3) picks the array's region of the staging buffer */
 static int up(slot_t* s, void* d, const double* h, int k) {
   const size_t n = (size_t)pacg_plan_size(s->plan);
@@ -120,6 +276,7 @@ static int up(slot_t* s, void* d, const double* h, int k) {
     memcpy(st, h, n * sizeof(double));
     return pacg_memcpy_h2d(d, st, n * sizeof(double), pacg_plan_stream(s->plan));
   }
+  if (ring_ready()) return ring_h2d(d, h, n * sizeof(double), pacg_plan_stream(s->plan));
   return pacg_memcpy_h2d(d, h, n * sizeof(double), pacg_plan_stream(s->plan));
 }
 /* device array d -> host array h; staged slots land in region k of the staging
@@ -127,6 +284,7 @@ static int up(slot_t* s, void* d, const double* h, int k) {
 static int down(slot_t* s, double* h, const void* d, int k) {
   const size_t n = (size_t)pacg_plan_size(s->plan);
   if (s->stage) return pacg_memcpy_d2h(s->stage + (size_t)k * n, d, n * sizeof(double), pacg_plan_stream(s->plan));
+  if (ring_ready()) return ring_d2h(h, d, n * sizeof(double), pacg_plan_stream(s->plan));
   return pacg_memcpy_d2h(h, d, n * sizeof(double), pacg_plan_stream(s->plan));
 }
 static void finish_down(slot_t* s, double* h, int k) {
@@ -256,6 +414,8 @@ void pacg_dropin_release(void) {
   pthread_mutex_lock(&g_mu);
   for (int i = 0; i < NSLOT; ++i)
     if (g_slot[i].plan) slot_free(&g_slot[i]);
+  ring_release();
+  for (int i = 0; i < RING_SLOTS; ++i) g_ring_used[i] = 0;
   pthread_mutex_unlock(&g_mu);
 }
 

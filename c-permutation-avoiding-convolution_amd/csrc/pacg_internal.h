@@ -189,6 +189,7 @@ const char* pacg_hip_error_string(int code);
 int pacg_event_create(void** ev);
 int pacg_event_destroy(void* ev);
 int pacg_event_record(void* ev, void* stream);
+int pacg_event_sync(void* ev);
 int pacg_event_elapsed_ms(float* ms, void* start, void* stop);
 int pacg_graph_begin(void* stream);
 int pacg_graph_end(void* stream, void** exec);

@@ -160,6 +160,39 @@ def test_dropin_against_reference_golden(gpu, name):
             assert rel_l2(join(xr, xi), g[key]) < TOL64, kind
 
 
+def _bitrev(t):
+    i = np.arange(1 << t)
+    out = np.zeros_like(i)
+    for b in range(t):
+        out |= ((i >> b) & 1) << (t - 1 - b)
+    return out
+
+
+@pytest.mark.parametrize("t", [12, 17, 18, 20, 22])
+def test_dropin_copy_paths(gpu, t):
+    """Host arrays of every copy path of the drop-in entry points: up to 2^17 doubles through the
+    pinned staging buffer, above that through the pinned ring in 8 MiB chunks (2^18: one partial
+    chunk, 2^20: one full chunk, 2^22: four chunks, so ring slots are reused within one array)."""
+    n = 1 << t
+    g = np.random.default_rng(40 + t)
+    x = g.uniform(-1, 1, n) + 1j * g.uniform(-1, 1, n)
+    h = g.uniform(-1, 1, n) + 1j * g.uniform(-1, 1, n)
+    Fh = np.fft.fft(h)
+    y_ref = np.fft.ifft(np.fft.fft(x) * Fh)
+    rho = _bitrev(t).astype(np.int32)
+    for _ in range(2):  # the second call reuses the cached plan, staging buffer and ring
+        xr, xi = np.ascontiguousarray(x.real), np.ascontiguousarray(x.imag)
+        hr, hi = np.ascontiguousarray(h.real), np.ascontiguousarray(h.imag)
+        gpu.permutation_avoiding_convolution(xr, xi, n, 2, None, None, hr, hi)
+        assert rel_l2(xr + 1j * xi, y_ref) < TOL64
+        assert rel_l2(hr + 1j * hi, Fh[rho]) < TOL64  # h <- A^T h = P F h
+        xr, xi = np.ascontiguousarray(x.real), np.ascontiguousarray(x.imag)
+        hr, hi = np.ascontiguousarray(h.real), np.ascontiguousarray(h.imag)
+        gpu.fft_based_convolution(xr, xi, n, 2, None, None, rho, hr, hi)
+        assert rel_l2(xr + 1j * xi, y_ref) < TOL64
+        assert rel_l2(hr + 1j * hi, Fh) < TOL64  # h <- F h
+
+
 def test_dropin_rejects_bad_input(gpu):
     x = np.zeros(24)
     y = np.zeros(24)
