@@ -11,7 +11,9 @@
  * The caller's twiddle tables are not read (the device builds its own FP64
  * tables); the caller's rho is uploaded and applied as a gather.
  * On invalid input nothing is modified and pacg_last_error() says why.
- * Calls are serialised by one mutex; each is synchronous.
+ * Calls are serialised by one mutex; each is synchronous.  Host arrays move
+ * through pinned memory: a staging buffer per cached shape up to 1 MiB per
+ * array, a shared chunked ring (host copies on a small thread pool) above.
  */
 #define _GNU_SOURCE
 #include <pthread.h>
