@@ -11,8 +11,8 @@
 // 1024 threads takes items blockIdx.x, blockIdx.x + gridDim.x, ... until none
 // are left (every wave reaches the same exit).  Within an item, the n / E
 // threads of one convolution each hold E points per register phase; the
-// phases exchange through LDS (at most 2^14 points, 128 KB; slots swizzled
-// by lds_swz, so no exchange meets a bank conflict).
+// phases exchange through LDS (at most 2^14 points, 128 KB of (re, im) float
+// pairs, one 8-byte access per point; slots swizzled by lds_swz).
 // Schedules per length come from a variant table the autotuner picks from.
 #include <hip/hip_runtime.h>
 
@@ -25,7 +25,7 @@ namespace pacg {
 
 constexpr int BT = 1024;              // threads per workgroup
 constexpr int BPTS = 1 << 14;         // points per item
-constexpr int BLD = BPTS;             // LDS length per array (swizzled slots, no padding)
+constexpr int BLD = BPTS;             // LDS slots of (re, im) pairs (swizzled, no padding)
 
 // Phase twiddle of phase PH.  The R points of a butterfly group take
 // w^(mu * kr), kr = 0 .. R-1, with w = w_P^(2^sh); only the base b = w^mu
@@ -77,8 +77,8 @@ __device__ __forceinline__ void batch_phase_twiddle(float* vr, float* vi, int tl
 }
 
 template <class SH>
-__device__ __forceinline__ void batch_item(const pacg_batch_args_t& a, const pacg_batch_item_t it, float* lre,
-                                           float* lim, const float2* twh, const float2* twl_) {
+__device__ __forceinline__ void batch_item(const pacg_batch_args_t& a, const pacg_batch_item_t it, float2* lds,
+                                           const float2* twh, const float2* twl_) {
   constexpr int P = SH::P, E = SH::E, TPL = SH::TPL, NP = SH::NP;
   static_assert(BT % TPL == 0, "threads per convolution (BT / TPL convolutions per item)");
   // Recompute every per-thread index inside the persistent loop: hoisted out
@@ -127,8 +127,7 @@ __device__ __forceinline__ void batch_item(const pacg_batch_args_t& a, const pac
       sfor<RF>([&](auto r_) PACG_INL {
         constexpr int r = decltype(r_)::value;
         const int l = li(FROM_, c, r);
-        lre[l] = vr[c * RF + r];
-        lim[l] = vi[c * RF + r];
+        lds[l] = make_float2(vr[c * RF + r], vi[c * RF + r]);
       });
     });
     __syncthreads();
@@ -137,8 +136,9 @@ __device__ __forceinline__ void batch_item(const pacg_batch_args_t& a, const pac
       sfor<RT>([&](auto r_) PACG_INL {
         constexpr int r = decltype(r_)::value;
         const int l = li(TO_, c, r);
-        vr[c * RT + r] = lre[l];
-        vi[c * RT + r] = lim[l];
+        const float2 w = lds[l];
+        vr[c * RT + r] = w.x;
+        vi[c * RT + r] = w.y;
       });
     });
   };
@@ -204,8 +204,8 @@ __device__ __forceinline__ void batch_item(const pacg_batch_args_t& a, const pac
 // thread that later holds the same positions of x (phase 0 of virtual thread
 // v, for both), so it stays there: 2 x VT x 8 values, no global round trip.
 template <class SH>
-__device__ __forceinline__ void batch_item_vt(const pacg_batch_args_t& a, const pacg_batch_item_t it, float* lre,
-                                              float* lim, const float2* twh, const float2* twl_) {
+__device__ __forceinline__ void batch_item_vt(const pacg_batch_args_t& a, const pacg_batch_item_t it, float2* lds,
+                                              const float2* twh, const float2* twl_) {
   constexpr int P = SH::P, E = SH::E, NP = SH::NP, TPL = SH::TPL;
   constexpr int G = BPTS / P;
   constexpr int VT = G * TPL / BT;
@@ -249,8 +249,9 @@ __device__ __forceinline__ void batch_item_vt(const pacg_batch_args_t& a, const 
           vi[c * R + r] = valid ? gi[base + sp] : 0.f;
         } else {
           const int l = lg + (lds_swz(pos<SH, PH>(tl, 0, 0)) ^ lds_swz(pos<SH, PH>(0, c, r)));
-          vr[c * R + r] = lre[l];
-          vi[c * R + r] = lim[l];
+          const float2 w = lds[l];
+          vr[c * R + r] = w.x;
+          vi[c * R + r] = w.y;
         }
       });
     });
@@ -291,8 +292,7 @@ __device__ __forceinline__ void batch_item_vt(const pacg_batch_args_t& a, const 
           }
         } else {
           const int l = lg + (lds_swz(pos<SH, PH>(tl, 0, 0)) ^ lds_swz(pos<SH, PH>(0, c, r)));
-          lre[l] = vr[c * R + r];
-          lim[l] = vi[c * R + r];
+          lds[l] = make_float2(vr[c * R + r], vi[c * R + r]);
         }
       });
     });
@@ -356,8 +356,7 @@ __device__ __forceinline__ void batch_item_vt(const pacg_batch_args_t& a, const 
   X(14, 0, 5, 0x3FEu, 5, 0x43210u)    /* 4 | 8 | 8 | 8 | 8 (two 8-point virtual threads) */
 
 __global__ void __launch_bounds__(BT) batch_kernel(const pacg_batch_args_t a) {
-  __shared__ float lre[BLD];
-  __shared__ float lim[BLD];
+  __shared__ float2 lds[BLD];
   __shared__ float2 twh[128], twl[128];
   {
     const float2* tw = reinterpret_cast<const float2*>(a.tw);
@@ -374,9 +373,9 @@ __global__ void __launch_bounds__(BT) batch_kernel(const pacg_batch_args_t a) {
   case M * 4 + V:                                                          \
     static_assert(Shape<NS, SC, NPH, PC>::LGP == M, "variant length");    \
     if constexpr (Shape<NS, SC, NPH, PC>::P >= BT * 8)                     \
-      batch_item_vt<Shape<NS, SC, NPH, PC>>(a, it, lre, lim, twh, twl);    \
+      batch_item_vt<Shape<NS, SC, NPH, PC>>(a, it, lds, twh, twl);    \
     else                                                                   \
-      batch_item<Shape<NS, SC, NPH, PC>>(a, it, lre, lim, twh, twl);       \
+      batch_item<Shape<NS, SC, NPH, PC>>(a, it, lds, twh, twl);       \
     break;
       PACG_BATCH_VARIANTS(PACG_CASE)
 #undef PACG_CASE
