@@ -809,15 +809,26 @@ int pacg_plan_autotune(pacg_plan* pl, const void* xr, const void* xi, void* yr, 
     int vsp = kind == PACG_K_FUSED ? sp : prec;
     const pacg_variant_t* cur = ps->v[prec][kind][vsp];
     if (!cur) continue;
-    const pacg_variant_t* best = cur;
-    float best_ms = 1e30f;
-    /* tables: the generated pass kernels, then the MFMA pass kernels */
-    for (int t = 0; t <= pacg_num_variant_tables && !rc; ++t) {
+    /* candidates: the generated pass kernels, then the MFMA pass kernels, of the same pass */
+    enum { MAXC = 32 };
+    const pacg_variant_t* cand[MAXC];
+    float cms[MAXC];
+    int nc = 0, def = -1;
+    for (int t = 0; t <= pacg_num_variant_tables && nc < MAXC; ++t) {
       const pacg_variant_t* tab = t < pacg_num_variant_tables ? pacg_variant_tables[t] : pacg_mfma_variants;
       const int cnt = t < pacg_num_variant_tables ? pacg_variant_counts[t] : pacg_num_mfma_variants;
-      for (int j = 0; j < cnt && !rc; ++j) {
-        const pacg_variant_t* v = &tab[j];
-        if (!same_pass(v, cur)) continue;
+      for (int j = 0; j < cnt && nc < MAXC; ++j) {
+        if (!same_pass(&tab[j], cur)) continue;
+        if (tab[j].rank == 0 && tab[j].engine == cur->engine) def = nc;
+        cms[nc] = 1e30f;
+        cand[nc++] = &tab[j];
+      }
+    }
+    /* two rounds over the candidates, each keeps its faster round: a drift of the
+     * clock during tuning then penalises no candidate for its place in the list */
+    for (int round = 0; round < 2 && !rc; ++round) {
+      for (int c = 0; c < nc && !rc; ++c) {
+        const pacg_variant_t* v = cand[c];
         ps->v[prec][kind][vsp] = v;
         const void* s_re = kind == PACG_K_FUSED ? pl->spec_re : NULL;
         const void* s_im = kind == PACG_K_FUSED ? pl->spec_im : NULL;
@@ -833,11 +844,17 @@ int pacg_plan_autotune(pacg_plan* pl, const void* xr, const void* xi, void* yr, 
         }
         float ms = 0.0f;
         pacg_event_elapsed_ms(&ms, e0, e1);
-        if (ms < best_ms) {
-          best_ms = ms;
-          best = v;
-        }
+        if (ms < cms[c]) cms[c] = ms;
       }
+    }
+    /* the fastest; the default tile unless another is faster by more than 2 % */
+    const pacg_variant_t* best = cur;
+    if (nc > 0) {
+      int b = 0;
+      for (int c = 1; c < nc; ++c)
+        if (cms[c] < cms[b]) b = c;
+      if (def >= 0 && cms[def] <= 1.02f * cms[b]) b = def;
+      best = cand[b];
     }
     ps->v[prec][kind][vsp] = best;
     if (best != cur) ++changed;
