@@ -116,17 +116,22 @@ __device__ __forceinline__ void batch_item(const pacg_batch_args_t& a, const pac
       });
     });
   };
-  auto exchange = [&](auto FROM_, auto TO_) PACG_INL {
+  // Exchange number X of the item goes through LDS buffer X % 2 (an item holds
+  // at most 1024 * 8 points, so two buffers fit): the buffer it writes was last
+  // read two exchanges ago, before the barrier of the exchange in between, so
+  // one barrier per exchange suffices.
+  auto exchange = [&](auto FROM_, auto TO_, auto X_) PACG_INL {
     constexpr int FROM = decltype(FROM_)::value;
     constexpr int TO = decltype(TO_)::value;
     constexpr int RF = 1 << SH::lgR(FROM);
     constexpr int RT = 1 << SH::lgR(TO);
-    __syncthreads();
+    constexpr int BUF = (decltype(X_)::value & 1) * (BPTS / 2);
+    static_assert(BT * E <= BPTS / 2, "two exchange buffers per item");
     sfor<E / RF>([&](auto c_) PACG_INL {
       constexpr int c = decltype(c_)::value;
       sfor<RF>([&](auto r_) PACG_INL {
         constexpr int r = decltype(r_)::value;
-        const int l = li(FROM_, c, r);
+        const int l = BUF + li(FROM_, c, r);
         lds[l] = make_float2(vr[c * RF + r], vi[c * RF + r]);
       });
     });
@@ -135,17 +140,19 @@ __device__ __forceinline__ void batch_item(const pacg_batch_args_t& a, const pac
       constexpr int c = decltype(c_)::value;
       sfor<RT>([&](auto r_) PACG_INL {
         constexpr int r = decltype(r_)::value;
-        const int l = li(TO_, c, r);
+        const int l = BUF + li(TO_, c, r);
         const float2 w = lds[l];
         vr[c * RT + r] = w.x;
         vi[c * RT + r] = w.y;
       });
     });
   };
-  auto dif = [&]() PACG_INL {
+  auto dif = [&](auto X0_) PACG_INL {  // X0: number of the first exchange
     sfor<NP>([&](auto i_) PACG_INL {
       constexpr int PH = NP - 1 - decltype(i_)::value;
-      if constexpr (PH < NP - 1) exchange(std::integral_constant<int, PH + 1>{}, std::integral_constant<int, PH>{});
+      if constexpr (PH < NP - 1)
+        exchange(std::integral_constant<int, PH + 1>{}, std::integral_constant<int, PH>{},
+                 std::integral_constant<int, decltype(X0_)::value + NP - 2 - PH>{});
       phase_stages<float, SH, PH, M_DIF>(vr, vi);
       batch_phase_twiddle<SH, PH, false>(vr, vi, tl, twh, twl_);
     });
@@ -153,7 +160,7 @@ __device__ __forceinline__ void batch_item(const pacg_batch_args_t& a, const pac
 
   load_top(a.kr, a.ki, vr, vi);
   load_top(a.xr, a.xi, pr, pi);  // x arrives while k is transformed
-  dif();                         // A^T k
+  dif(std::integral_constant<int, 0>{});  // A^T k
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     sr[e] = vr[e];
@@ -161,7 +168,7 @@ __device__ __forceinline__ void batch_item(const pacg_batch_args_t& a, const pac
     vr[e] = pr[e];
     vi[e] = pi[e];
   }
-  dif();  // A^T x
+  dif(std::integral_constant<int, NP - 1>{});  // A^T x
   const float scale = 1.0f / (float)P;
 #pragma unroll
   for (int e = 0; e < E; ++e) {
@@ -171,7 +178,9 @@ __device__ __forceinline__ void batch_item(const pacg_batch_args_t& a, const pac
   }
   sfor<NP>([&](auto i_) PACG_INL {
     constexpr int PH = decltype(i_)::value;
-    if constexpr (PH > 0) exchange(std::integral_constant<int, PH - 1>{}, std::integral_constant<int, PH>{});
+    if constexpr (PH > 0)
+      exchange(std::integral_constant<int, PH - 1>{}, std::integral_constant<int, PH>{},
+               std::integral_constant<int, 2 * (NP - 1) + PH - 1>{});
     batch_phase_twiddle<SH, PH, true>(vr, vi, tl, twh, twl_);
     phase_stages<float, SH, PH, M_DITC>(vr, vi);
   });
