@@ -378,6 +378,21 @@ def bench_mfma(pacg, synth, steps, warmup):
     return out
 
 
+def bench_hbm_probe(pacg):
+    """pacg_hbm_probe on 512 MiB buffers: the rates a plain streaming kernel reaches (read, write, copy;
+    bytes moved per second, a copy counting its reads and its writes), beside the 8 TB/s peak."""
+    nb = 512 << 20
+    src = pacg.DeviceArray(nb // 4, np.float32)
+    dst = pacg.DeviceArray(nb // 4, np.float32)
+    src.upload(np.zeros(nb // 4, np.float32))
+    out = {"bytes": nb}
+    for name, mode, moved in (("read", 0, nb), ("write", 1, nb), ("copy", 2, 2 * nb)):
+        ms = pacg.hbm_probe(mode, dst, src, nb, reps=10)
+        out[name + "_GBs"] = moved / (ms * 1e-3) / 1e9
+    del src, dst
+    return out
+
+
 def bench_batched(pacg, synth, steps, warmup):
     """4096 FP32 pairs of lengths 2^U{8..14} (seed 4) in one persistent, graph-captured launch,
     against per-length plain FP64 launches (one batch-axis plan per length)."""
@@ -645,6 +660,10 @@ def main():
         log("batched workload ...")
         out["batched"] = bench_batched(pacg, synth, a.steps, a.warmup)
         log("batched: %.3f ms per batch of 4096" % out["batched"]["ms_per_batch"])
+    log("hbm probe ...")
+    probe = bench_hbm_probe(pacg)
+    probe["fused_kernel_over_copy"] = out["roofline"]["achieved"] / probe["copy_GBs"]
+    out["hbm_probe"] = probe
     if not a.no_cpu:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(x, k, a.cpu_reps)

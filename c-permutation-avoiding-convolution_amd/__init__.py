@@ -93,6 +93,7 @@ def _load():
     L.pacg_plan_launch_candidates.argtypes = [_vp, C.c_int, C.POINTER(C.c_char_p), C.c_int]
     L.pacg_plan_pin_launch.argtypes = [_vp, C.c_int, C.c_char_p]
     L.pacg_bw_probe.argtypes = [C.c_int, _vp, _vp, C.c_uint64, _vp]
+    L.pacg_hbm_probe.argtypes = [C.c_int, _vp, _vp, C.c_uint64, C.c_int, _vp, C.POINTER(C.c_float)]
     L.pacg_dropin_release.argtypes = []
     L.pacg_dropin_release.restype = None
     L.pacg_desc_init.argtypes = [C.POINTER(_Desc), C.c_int, _ip, C.c_int, C.c_int]
@@ -362,8 +363,19 @@ class Plan:
 
 
 def bw_probe(mode: int, dst: DeviceArray, src: DeviceArray, nbytes: int, stream=None):
-    """HBM probe kernel: mode 0 read, 1 write, 2 copy (16-byte accesses)."""
+    """HBM probe kernel: mode 0 read, 1 write, 2 copy (16-byte accesses), one launch, no timing."""
     _check(lib.pacg_bw_probe(int(mode), dst.ptr, src.ptr, int(nbytes), stream), "bw probe")
+
+
+def hbm_probe(mode: int, dst, src, nbytes: int, reps: int = 10, stream=None) -> float:
+    """pacg_hbm_probe: mean ms of one probe launch (mode 0 read, 1 write, 2 copy of nbytes);
+    dst / src are DeviceArray objects (or None where the mode does not use them)."""
+    ms = C.c_float(0.0)
+    rc = lib.pacg_hbm_probe(int(mode), dst.ptr if dst is not None else None, src.ptr if src is not None else None,
+                            C.c_uint64(int(nbytes)), int(reps), stream, C.byref(ms))
+    if rc != 0:
+        raise PacgError("pacg_hbm_probe: %s" % last_error())
+    return ms.value
 
 
 class Batch:

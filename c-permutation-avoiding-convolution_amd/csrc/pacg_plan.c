@@ -518,6 +518,34 @@ int pacg_plan_set_stream(pacg_plan* pl, void* stream) {
 }
 void* pacg_plan_stream(const pacg_plan* pl) { return pl ? pl->stream : NULL; }
 long long pacg_plan_size(const pacg_plan* pl) { return pl ? pl->n : 0; }
+int pacg_hbm_probe(int mode, void* dst, const void* src, uint64_t bytes, int reps, void* stream, float* ms) {
+  if (mode < 0 || mode > 2) return pacg_fail(PACG_ERR_ARG, "pacg_hbm_probe: mode %d not in 0..2", mode);
+  if (bytes == 0 || bytes % 16 != 0)
+    return pacg_fail(PACG_ERR_ARG, "pacg_hbm_probe: %llu bytes is not a positive multiple of 16",
+                     (unsigned long long)bytes);
+  if (reps < 1 || !ms || (mode != 1 && !src) || (mode != 0 && !dst))
+    return pacg_fail(PACG_ERR_ARG, "pacg_hbm_probe: bad argument");
+  void *e0 = NULL, *e1 = NULL;
+  int rc = PACG_OK;
+  if (pacg_event_create(&e0) || pacg_event_create(&e1)) {
+    rc = pacg_fail(PACG_ERR_HIP, "pacg_hbm_probe: event creation failed");
+  } else if (pacg_bw_probe(mode, dst, src, bytes, stream)) { /* warm-up */
+    rc = pacg_fail(PACG_ERR_HIP, "pacg_hbm_probe: launch failed");
+  } else {
+    int e = pacg_event_record(e0, stream);
+    for (int r = 0; r < reps && !e; ++r) e |= pacg_bw_probe(mode, dst, src, bytes, stream);
+    e |= pacg_event_record(e1, stream);
+    e |= pacg_event_sync(e1);
+    if (!e) e |= pacg_event_elapsed_ms(ms, e0, e1);
+    if (e) rc = pacg_fail(PACG_ERR_HIP, "pacg_hbm_probe: timing failed (%d)", e);
+    else *ms /= (float)reps;
+  }
+  if (e0) pacg_event_destroy(e0);
+  if (e1) pacg_event_destroy(e1);
+  if (rc == PACG_OK) pacg_clear_error();
+  return rc;
+}
+
 int pacg_sync(pacg_plan* pl) {
   if (!pl) return pacg_fail(PACG_ERR_ARG, "NULL plan");
   HIPCHK(pacg_stream_sync(pl->stream), "stream synchronize");

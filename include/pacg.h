@@ -22,6 +22,8 @@
 #ifndef PACG_H
 #define PACG_H
 
+#include <stdint.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -161,6 +163,15 @@ int pacg_batch_sync(pacg_batch* batch);
 
 /* Free the device plans and buffers cached by the drop-in fft.h entry points. */
 void pacg_dropin_release(void);
+
+/* HBM probe, the device counterpart of the reference's host bandwidth tool
+ * (memory_bw_test.c:27-67, read / write / copy of large arrays): mode 0 reads
+ * `bytes` of src, 1 writes `bytes` to dst, 2 copies src to dst, with 16-byte
+ * accesses from a grid that fills the device.  `reps` launches are timed with
+ * HIP events on `stream` (NULL: the default stream); *ms receives the mean time
+ * of one launch.  bytes must be a positive multiple of 16 and src / dst device
+ * buffers of at least that size (src unused by mode 1, dst by mode 0). */
+int pacg_hbm_probe(int mode, void* dst, const void* src, uint64_t bytes, int reps, void* stream, float* ms);
 
 /* Wait for the plan stream. */
 int pacg_sync(pacg_plan* plan);

@@ -130,6 +130,19 @@ def test_plan_validation_errors_need_no_gpu(pacg):
         pacg.Plan([64], radix=3)
 
 
+def test_hbm_probe_validation_needs_no_gpu(pacg):
+    with pytest.raises(pacg.PacgError, match="mode 3"):
+        pacg.hbm_probe(3, None, None, 1 << 20)
+    with pytest.raises(pacg.PacgError, match="multiple of 16"):
+        pacg.hbm_probe(2, None, None, 24)
+    with pytest.raises(pacg.PacgError, match="multiple of 16"):
+        pacg.hbm_probe(0, None, None, 0)
+    with pytest.raises(pacg.PacgError, match="bad argument"):
+        pacg.hbm_probe(2, None, None, 1 << 20)  # copy needs both buffers
+    with pytest.raises(pacg.PacgError, match="bad argument"):
+        pacg.hbm_probe(1, None, None, 1 << 20, reps=0)
+
+
 def test_dropin_validation_errors_need_no_gpu(pacg):
     x = np.zeros(24)
     y = np.zeros(24)
