@@ -459,12 +459,14 @@ __global__ void __launch_bounds__(LINES* SH::TPL)
   constexpr int LDP = LMODE == LM_LINESWZ ? P : P + P / 8 + 1;
   constexpr int LDS_N = (LAYOUT == PACG_L_STRIDED) ? P * LDJ : LINES * LDP;
   constexpr bool NEED_X = NP > 1;
-  // FP64 contiguous lines of 2048+ points exchange the real and the imaginary
-  // parts one after the other through one LDS array: half the LDS per line, so
-  // twice the workgroups per CU (2 -> 4 waves per SIMD at 2048 points, where
-  // LDS and not registers bounded the occupancy; 44.5 -> 37.6 us per pass at
-  // 2048^2).  Tiles bounded by registers gain nothing and keep one exchange.
-  constexpr bool SPLIT = !F32 && LAYOUT == PACG_L_CONTIG && P >= 2048;
+  // FP64 contiguous lines of 2048+ points with 16 points per thread exchange
+  // the real and the imaginary parts one after the other through one LDS
+  // array: half the LDS per line, so twice the workgroups per CU (2 -> 4 waves
+  // per SIMD, where LDS and not registers bounded the occupancy; 44.5 -> 37.6
+  // us per pass at 2048^2).  With 8 points per thread (the default for these
+  // lines) the paired exchange below is faster (about 31 against 33 us), and
+  // tiles bounded by registers gain nothing from the split.
+  constexpr bool SPLIT = !F32 && LAYOUT == PACG_L_CONTIG && P >= 2048 && E > 8;
   // The tiles hold (re, im) pairs: one 8- or 16-byte LDS access per point and
   // per twiddle instead of two (the slot indices of the layouts above are
   // unchanged).  At 256^3 FP32 the fused pass goes from 79.5 to 76 us, with
