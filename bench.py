@@ -465,6 +465,9 @@ def main():
                     help="--profile: the pair-launch plan instead of one axis per launch")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched uneven-length workload")
     ap.add_argument("--no-mfma", action="store_true", help="skip the 4096x256 MFMA workload")
+    ap.add_argument("--profile-fp64-2d", action="store_true",
+                    help="only run the 2D 2048^2 FP64 plan (radix 4, one radix-2 stage per axis), autotuned, "
+                         "`steps` times (for rocprofv3)")
     ap.add_argument("--profile-mfma", action="store_true",
                     help="only run the 4096x256 MFMA convolution `steps` times (for rocprofv3 counters)")
     ap.add_argument("--profile-batched", action="store_true",
@@ -494,6 +497,20 @@ def main():
             b.replay()
         b.sync()
         print(json.dumps({"profile_batched": True, "steps": a.steps, "points": int(b.total)}))
+        return
+
+    if a.profile_fp64_2d:
+        x2, k2 = synth.workload_inputs("2d_2048x2048")
+        pl = pacg.Plan(x2.shape, schedules=[[4, 4, 4, 4, 4, 2]] * 2, precision=pacg.PREC_F64)
+        pl.set_kernel(k2)
+        dx = pacg.DeviceComplex.from_numpy(x2)
+        dy = pl.alloc()
+        pl.autotune(dx, dy, 3)
+        pl.sync()
+        for _ in range(a.steps):
+            pl.convolve(dx, dy)
+        pl.sync()
+        print(json.dumps({"profile_fp64_2d": True, "steps": a.steps, "plan": pl.describe()}))
         return
 
     if a.profile_mfma:
