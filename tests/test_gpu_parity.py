@@ -193,6 +193,23 @@ def test_dropin_copy_paths(gpu, t):
         assert rel_l2(hr + 1j * hi, Fh) < TOL64  # h <- F h
 
 
+@pytest.mark.parametrize("dims,r", [((1 << 24,), 4), ((1 << 21,), 8), ((1024, 4, 4), 4), ((4, 4, 2048), 2)])
+def test_dropin_large_and_anisotropic(gpu, dims, r):
+    """Drop-in convolution at large 1D sizes (2^24 = 4^12 points, 2^21 = 8^7) and on anisotropic 3D
+    shapes like those of the reference's *_anisotropic driver programs (two axes of 4 points),
+    against numpy.fft."""
+    n = int(np.prod(dims))
+    g = np.random.default_rng(60 + len(dims) + r)
+    x = (g.uniform(-1, 1, n) + 1j * g.uniform(-1, 1, n)).reshape(dims)
+    h = (g.uniform(-1, 1, n) + 1j * g.uniform(-1, 1, n)).reshape(dims)
+    y_ref = np.fft.ifftn(np.fft.fftn(x) * np.fft.fftn(h))
+    xr, xi = np.ascontiguousarray(x.real.ravel()), np.ascontiguousarray(x.imag.ravel())
+    hr, hi = np.ascontiguousarray(h.real.ravel()), np.ascontiguousarray(h.imag.ravel())
+    sfx = {1: "", 2: "_2d", 3: "_3d"}[len(dims)]
+    getattr(gpu, "permutation_avoiding_convolution" + sfx)(xr, xi, *dims, r, *([None] * (2 * len(dims))), hr, hi)
+    assert rel_l2((xr + 1j * xi).reshape(dims), y_ref) < TOL64
+
+
 def test_dropin_rejects_bad_input(gpu):
     x = np.zeros(24)
     y = np.zeros(24)
