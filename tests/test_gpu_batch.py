@@ -116,3 +116,32 @@ def test_ragged_items_of_8192_points(gpu, oracle):
     b.convolve(dx, dk, dy)
     b.sync()
     check(b, xs, ks, dy, oracle, {m: gpu.Batch.schedule(m, b.variant(m)) for m in range(8, 15)})
+
+
+def test_full_size_batch_of_4096_pairs(gpu, synth):
+    """The benchmark batch itself (4096 pairs, lengths 2^U{8..14}, seed 4), autotuned and replayed
+    from its graph: every pair of every length against numpy.fft in complex128, per pair and over the
+    whole batch."""
+    xs, ks = synth.batched_inputs(4, 4096)
+    ms = np.array([int(np.log2(len(x))) for x in xs], dtype=np.int32)
+    b = gpu.Batch(ms)
+    dx = gpu.DeviceComplex.from_numpy(b.pack(xs), np.float32)
+    dk = gpu.DeviceComplex.from_numpy(b.pack(ks), np.float32)
+    dy = b.alloc()
+    b.autotune(dx, dk, dy, 2)
+    b.capture(dx, dk, dy)
+    for _ in range(3):
+        dy.upload(np.zeros(int(b.total), complex))
+        b.replay()
+        b.sync()
+        ys = b.unpack(dy.numpy())
+        num = den = 0.0
+        worst = 0.0
+        for x, k, y in zip(xs, ks, ys):
+            ref = np.fft.ifft(np.fft.fft(x) * np.fft.fft(k))
+            d = y - ref
+            num += float(np.vdot(d, d).real)
+            den += float(np.vdot(ref, ref).real)
+            worst = max(worst, rel_l2(y, ref))
+        assert worst < TOL32, worst
+        assert (num / den) ** 0.5 < 1e-6
