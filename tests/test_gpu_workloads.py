@@ -82,3 +82,25 @@ def test_3d_256cubed(gpu, synth, prec, tol, scheds, pairs, launches):
     pl.set_kernel(delta)
     pl.convolve(dx, dy)
     assert rel_l2(dy.numpy(), x) < tol
+
+
+def test_3d_256cubed_shift_and_linearity(gpu, synth):
+    """Headline plan (FP32, 4-8-8 per axis) at full size against closed forms: a shifted delta
+    kernel rolls x by its offset on every axis, and the convolution is linear in x."""
+    x, _ = synth.workload_inputs("3d_256cubed")
+    pl = gpu.Plan(x.shape, schedules=[[4, 8, 8]] * 3, precision=1, spectrum_precision=1)
+    shift = (17, 200, 3)
+    k = np.zeros(x.shape, complex)
+    k[shift] = 1.0
+    pl.set_kernel(k)
+    dx = gpu.DeviceComplex.from_numpy(x, np.float32)
+    dy = pl.alloc()
+    pl.convolve(dx, dy)
+    assert rel_l2(dy.numpy(), np.roll(x, shift, axis=(0, 1, 2))) < 2e-6
+    g = np.random.default_rng(21)
+    x2 = g.standard_normal(x.shape) + 1j * g.standard_normal(x.shape)
+    pl.convolve(gpu.DeviceComplex.from_numpy(x2, np.float32), dy)
+    y2 = dy.numpy()
+    pl.convolve(gpu.DeviceComplex.from_numpy(2.0 * x - x2, np.float32), dy)
+    y12 = dy.numpy()
+    assert rel_l2(y12, 2.0 * np.roll(x, shift, axis=(0, 1, 2)) - y2) < 2e-6
