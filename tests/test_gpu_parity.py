@@ -277,3 +277,39 @@ def test_unaligned_buffers_take_the_scalar_path(gpu, oracle, prec, tol):
     for op, kind in ((gpu.OP_TRANS, TRANS), (gpu.OP_CONJ, CONJ), (gpu.OP_FWD, A)):
         pl.execute(op, dx, dy)
         assert rel_l2(dy.numpy(), oracle.butterflies_nd(x, scheds, kind)) < tol
+
+
+def test_dropin_calls_from_several_threads(gpu):
+    """Drop-in calls from four host threads at once (ctypes releases the GIL): the calls serialise
+    on the library's mutex and share the pinned ring and copy pool, and every result matches."""
+    import threading
+
+    sizes = [1 << 14, 1 << 19, 1 << 20, 1 << 21]
+    g = np.random.default_rng(77)
+    cases = []
+    for n in sizes:
+        x = g.uniform(-1, 1, n) + 1j * g.uniform(-1, 1, n)
+        h = g.uniform(-1, 1, n) + 1j * g.uniform(-1, 1, n)
+        cases.append((n, x, h, np.fft.ifft(np.fft.fft(x) * np.fft.fft(h))))
+    errors = []
+
+    def work(case, reps):
+        n, x, h, y_ref = case
+        try:
+            for _ in range(reps):
+                xr, xi = np.ascontiguousarray(x.real), np.ascontiguousarray(x.imag)
+                hr, hi = np.ascontiguousarray(h.real), np.ascontiguousarray(h.imag)
+                gpu.permutation_avoiding_convolution(xr, xi, n, 2, None, None, hr, hi)
+                e = rel_l2(xr + 1j * xi, y_ref)
+                if not e < TOL64:
+                    errors.append((n, e))
+        except Exception as exc:  # surfaced below
+            errors.append((n, repr(exc)))
+
+    threads = [threading.Thread(target=work, args=(c, 3)) for c in cases]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join(timeout=120)
+    assert not any(t.is_alive() for t in threads)
+    assert not errors, errors
