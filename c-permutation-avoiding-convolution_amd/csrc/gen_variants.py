@@ -90,10 +90,17 @@ def tile_options(shape, prec, layout, et=E_TARGET):
             for alt in (sl // 2, sl // 4):
                 if alt >= 4 and alt * tpl >= 64:
                     opts.append(alt)
+            # lines of up to 256 points: narrower tiles, so that small arrays
+            # (a 2^16-point line has 256 lines per pass) still fill the CUs
+            if P <= 256:
+                for alt in (sl // 8, sl // 16):
+                    if 1 <= alt < 4 and alt * tpl >= 64:
+                        opts.append(alt)
     else:
         opts = [nl]
         if P >= 64:
-            for alt in (nl // 2, nl * 2):
+            alts = (nl // 2, nl * 2) + ((nl // 4,) if P <= 256 else ())
+            for alt in alts:
                 if 1 <= alt and alt * tpl <= 1024 and alt * tpl >= 64 and alt * (P + P // 8 + 1) * 2 * elem <= 80 * 1024:
                     opts.append(alt)
     return opts

@@ -146,6 +146,26 @@ static const pacg_variant_t* find_variant(int prec, int spec_prec, int kind, int
   return NULL;
 }
 
+static int same_pass(const pacg_variant_t* a, const pacg_variant_t* b);
+
+/* Start a pass on a narrower tile of the same pass when the default tile
+ * leaves CUs idle: of `lines` lines, the compiled tile with the most lines
+ * per workgroup that still gives FILL_BLOCKS workgroups, else the one that
+ * gives the most. */
+enum { FILL_BLOCKS = 256 };
+static const pacg_variant_t* fill_tile(const pacg_variant_t* v, long long lines) {
+  if (!v || (lines + v->lines - 1) / v->lines >= FILL_BLOCKS) return v;
+  const pacg_variant_t* best = v;
+  for (int t = 0; t < pacg_num_variant_tables; ++t)
+    for (int i = 0; i < pacg_variant_counts[t]; ++i) {
+      const pacg_variant_t* c = &pacg_variant_tables[t][i];
+      if (!same_pass(c, v)) continue;
+      const long long bc = (lines + c->lines - 1) / c->lines, bb = (lines + best->lines - 1) / best->lines;
+      if (bb >= FILL_BLOCKS ? (bc >= FILL_BLOCKS && c->lines > best->lines) : bc > bb) best = c;
+    }
+  return best;
+}
+
 static const pacg_pair_variant_t* find_pair(int prec, int spec_prec, int kind, int nsx, uint32_t codex, int nsy,
                                             uint32_t codey, int ycont, int max_u) {
   const pacg_pair_variant_t* best = NULL;
@@ -275,6 +295,8 @@ static int bind_variants(pacg_plan* pl) {
       long long S = (long long)ps->L * ax->inner;
       ps->layout = (S == 1) ? PACG_L_CONTIG : PACG_L_STRIDED;
       int ns = ps->q_hi - ps->q_lo + 1;
+      long long P = 1;
+      for (int q = ps->q_lo; q <= ps->q_hi; ++q) P *= ax->radix[q];
       for (int pr = 0; pr < 2; ++pr) {
         if (!pl->precs[pr]) continue;
         for (int k = 0; k < 4; ++k)
@@ -285,6 +307,7 @@ static int bind_variants(pacg_plan* pl) {
             } else {
               v = find_variant(pr, sp, k, ps->layout, ns, ps->code);
               if (!v && ps->layout == PACG_L_CONTIG) v = find_variant(pr, sp, k, PACG_L_STRIDED, ns, ps->code);
+              v = fill_tile(v, pl->n / P);
             }
             ps->v[pr][k][sp] = v;
           }
@@ -847,7 +870,7 @@ int pacg_plan_autotune(pacg_plan* pl, const void* xr, const void* xi, void* yr, 
       const int cnt = t < pacg_num_variant_tables ? pacg_variant_counts[t] : pacg_num_mfma_variants;
       for (int j = 0; j < cnt && nc < MAXC; ++j) {
         if (!same_pass(&tab[j], cur)) continue;
-        if (tab[j].rank == 0 && tab[j].engine == cur->engine) def = nc;
+        if (&tab[j] == cur) def = nc;
         cms[nc] = 1e30f;
         cand[nc++] = &tab[j];
       }
@@ -875,7 +898,7 @@ int pacg_plan_autotune(pacg_plan* pl, const void* xr, const void* xi, void* yr, 
         if (ms < cms[c]) cms[c] = ms;
       }
     }
-    /* the fastest; the default tile unless another is faster by more than 2 % */
+    /* the fastest; the tile the launch had unless another is faster by more than 2 % */
     const pacg_variant_t* best = cur;
     if (nc > 0) {
       int b = 0;

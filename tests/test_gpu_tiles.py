@@ -25,6 +25,10 @@ CASES = [
     ((64, 64, 256), [[8, 8], [8, 8], [4, 8, 8]], 0),
     # FP64 2048-point contiguous lines: split re/im exchange, 16- and 8-point phases
     ((64, 2048), [[8, 8], [4, 4, 4, 4, 4, 2]], 0),
+    # 1D 2^16 FP64 (the drop-in's radix-2 and radix-4 plans): 256 lines per pass, so the plan
+    # starts on the narrow tiles that give the most workgroups
+    ((65536,), [[2] * 16], 0),
+    ((65536,), [[4] * 8], 0),
 ]
 
 
