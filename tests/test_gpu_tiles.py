@@ -75,3 +75,17 @@ def test_pin_rejects_foreign_kernels(gpu):
         pl.pin(0, "f64_dif_c_8x8x4_h_l8")
     with pytest.raises(gpu.PacgError, match="no pass kernel"):
         pl.candidates(99)
+
+
+def test_small_arrays_start_on_tiles_that_fill_the_cus(gpu):
+    # 2^16 points in lines of 256: 256 lines per pass. The default tiles (16 and 8 lines per
+    # workgroup) would give 16 and 32 workgroups; the plan starts on the narrowest compiled ones.
+    pl = gpu.Plan((65536,), schedules=[[2] * 16], precision=gpu.PREC_F64)
+    desc = pl.describe()
+    assert "f64_dif_s_2x2x2x2x2x2x2x2_h_l4" in desc and "f64_dif_c_2x2x2x2x2x2x2x2_h_l4" in desc, desc
+    # the same 256-point pass over 65536 rows (65536 lines) keeps its default 8-line tile,
+    # over 256 rows it starts on the 4-line one
+    small = gpu.Plan((256, 256), schedules=["batch", [2] * 8], precision=gpu.PREC_F64).describe()
+    big = gpu.Plan((65536, 256), schedules=["batch", [2] * 8], precision=gpu.PREC_F64).describe()
+    assert "f64_dif_c_2x2x2x2x2x2x2x2_h_l4" in small, small
+    assert "f64_dif_c_2x2x2x2x2x2x2x2_h_l8" in big, big
