@@ -105,6 +105,7 @@ def _load():
     L.pacg_memcpy_d2d.argtypes = [_vp, _vp, C.c_size_t, _vp]
     L.pacg_stream_sync.argtypes = [_vp]
     L.pacg_device_count.argtypes = [_ip]
+    L.pacg_hip_error_string.argtypes = [C.c_int]
     L.pacg_hip_error_string.restype = C.c_char_p
     L.pacg_event_create.argtypes = [C.POINTER(_vp)]
     L.pacg_event_destroy.argtypes = [_vp]
@@ -507,6 +508,8 @@ _NULL_TW = None  # the GPU path does not read the caller's twiddle tables
 
 def _bfly(name, nd):
     fn = getattr(lib, name)
+    fn.argtypes = [_dp, _dp] + [C.c_int] * nd + [_vp] * (2 * nd)
+    fn.restype = None
 
     def call(re, im, *dims_and_tw):
         dims = [int(d) for d in dims_and_tw[:nd]]
@@ -531,6 +534,8 @@ for _r in (2, 4):
 
 def _transform(name, nd, with_rho):
     fn = getattr(lib, name)
+    fn.argtypes = [_dp, _dp] + [C.c_int] * (nd + 1) + [_vp] * (2 * nd) + ([_ip] if with_rho else [])
+    fn.restype = None
 
     def call(re, im, *args):
         dims = [int(d) for d in args[:nd]]
@@ -556,6 +561,8 @@ for _sfx, _nd in (("", 1), ("_2d", 2), ("_3d", 3)):
 
 def _conv(name, nd, with_rho):
     fn = getattr(lib, name)
+    fn.argtypes = [_dp, _dp] + [C.c_int] * (nd + 1) + [_vp] * (2 * nd) + ([_ip] if with_rho else []) + [_dp, _dp]
+    fn.restype = None
 
     def call(re, im, *args):
         dims = [int(d) for d in args[:nd]]

@@ -77,7 +77,6 @@ class Oracle:
         L.or_rev_index_mixed.argtypes = [C.c_long, C.c_long, _ip, C.c_int]
         L.or_rev_index_mixed.restype = C.c_long
         L.or_perm_nd.argtypes = [C.c_int, _lp, _ip, _ip, _i64p]
-        L.or_apply_permutation_split.argtypes = [_dp, _dp, C.c_long, _i64p]
         L.or_gather.argtypes = [_dp, _dp, C.c_long, _i64p]
         L.or_scatter.argtypes = [_dp, _dp, C.c_long, _i64p]
         L.or_twiddles_ref_layout.argtypes = [C.c_long, C.c_int, _dp, _dp]
@@ -87,10 +86,6 @@ class Oracle:
         L.or_fft_based_convolution_nd.argtypes = [_dp, _dp, _dp, _dp, C.c_int, _lp, _ip, _ip]
         L.or_max_error.argtypes = [_dp, _dp, _dp, C.c_long]
         L.or_max_error.restype = C.c_double
-        L.or_valid_schedule.argtypes = [C.c_long, _ip, C.c_int]
-        L.or_valid_schedule.restype = C.c_int
-        L.or_supports_radix_4.argtypes = [C.c_int]
-        L.or_supports_radix_8.argtypes = [C.c_int]
 
     # -- butterflies ---------------------------------------------------------
     def butterflies(self, x: np.ndarray, sched, kind: int) -> np.ndarray:
@@ -118,13 +113,6 @@ class Oracle:
         nd, d, s, ts = _dims_scheds(dims, scheds)
         self.L.or_perm_nd(nd, d, s, ts, out.ctypes.data_as(_i64p))
         return out
-
-    def apply_permutation_split(self, x: np.ndarray, rho: np.ndarray) -> np.ndarray:
-        re = np.ascontiguousarray(x.real, dtype=np.float64).copy().ravel()
-        im = np.ascontiguousarray(x.imag, dtype=np.float64).copy().ravel()
-        rho = np.ascontiguousarray(rho, dtype=np.int64)
-        self.L.or_apply_permutation_split(_d(re), _d(im), re.size, rho.ctypes.data_as(_i64p))
-        return (re + 1j * im).reshape(x.shape)
 
     def twiddles_ref_layout(self, n: int, r: int):
         wre = np.zeros(max(1, n), dtype=np.float64)

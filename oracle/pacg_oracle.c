@@ -13,7 +13,6 @@
  *   A^T           DIF stages q = t-1..0, F_r then * w^{mj}   butterflies.c:70-97,217-275,505-632
  *   2D/3D         per-axis application, contiguous axis first butterflies_2d.c:12-49, butterflies_3d.c:12-78
  *   digit reversal base-r, composed per axis                 index_reversals.c:13-72, _2d.c:14-62, _3d.c:14-72
- *   permutation   swap j <-> rho[j] where j < rho[j]         index_reversals.c:85-94
  *   fft/fft2/ufft/ifft/uifft                                 fft.c:12-88, fft_2d.c, fft_3d.c
  *   PA convolution  A^T x, A^T h, (x*h)/n, conj(A)           convolution.c:31-72, _2d.c:35-70, _3d.c:36-72
  *   FFT-based convolution                                    convolution.c:12-29, _2d.c:12-33, _3d.c:12-34
@@ -70,16 +69,6 @@ static void small_dft(int r, double* xr, double* xi, int conj) {
   }
   memcpy(xr, yr, (size_t)r * sizeof(double));
   memcpy(xi, yi, (size_t)r * sizeof(double));
-}
-
-int or_valid_schedule(long n, const int* sched, int t) {
-  if (n < 1 || t < 0 || t > OR_MAXT) return 0;
-  long p = 1;
-  for (int q = 0; q < t; ++q) {
-    if (sched[q] != 2 && sched[q] != 4 && sched[q] != 8) return 0;
-    p *= sched[q];
-  }
-  return p == n;
 }
 
 /* One stage q on a contiguous length-n vector (twiddle_factors.c:30-35 angle,
@@ -225,21 +214,6 @@ void or_perm_nd(int nd, const long* dims, const int* scheds, const int* ts, int6
   }
 }
 
-/* apply_permutation_split (index_reversals.c:85-94): swap j <-> rho[j] for j < rho[j].
- * Equals the gather x'[j] = x[rho[j]] whenever rho is an involution (pure radix). */
-void or_apply_permutation_split(double* re, double* im, long n, const int64_t* rho) {
-  for (long j = 0; j < n; ++j) {
-    long r = (long)rho[j];
-    if (j < r) {
-      double a = re[j], b = im[j];
-      re[j] = re[r];
-      im[j] = im[r];
-      re[r] = a;
-      im[r] = b;
-    }
-  }
-}
-
 /* Gather x'[p] = x[rho[p]] (the permutation P with A P = F, valid for mixed radix). */
 void or_gather(double* re, double* im, long n, const int64_t* rho) {
   double* tr = (double*)malloc((size_t)n * sizeof(double));
@@ -367,7 +341,3 @@ double or_max_error(const double* re, const double* im, const double* ref_flat, 
   }
   return mx;
 }
-
-/* supports_radix_4 / supports_radix_8 (misc.c:56-64). */
-int or_supports_radix_4(int t2) { return (t2 % 2) == 0; }
-int or_supports_radix_8(int t2) { return (t2 % 3) == 0; }

@@ -177,3 +177,110 @@ def test_batch_validation_needs_no_gpu(pacg):
         assert pacg.Batch.num_variants(m) >= 1
         for v in range(pacg.Batch.num_variants(m)):
             assert int(np.prod(pacg.Batch.schedule(m, v))) == 1 << m
+
+
+# ---------------------------------------------------------------------------
+# ctypes prototypes.  A function called through ctypes without argtypes gets
+# every Python int converted to a 32-bit C int: a 64-bit plan or device
+# pointer passed that way arrives truncated, and the library faults on the
+# host when it dereferences it.  Every C function the package or bench.py
+# calls must carry argtypes (and a restype for non-int results) whose sizes
+# match the prototype in include/ or csrc/pacg_internal.h.
+# ---------------------------------------------------------------------------
+_PROTO_HEADERS = HEADERS + [os.path.join(ROOT, "c-permutation-avoiding-convolution_amd", "csrc", "pacg_internal.h")]
+_INT_SIZES = {"int": 4, "unsigned": 4, "uint32_t": 4, "int32_t": 4, "uint64_t": 8, "int64_t": 8, "size_t": 8,
+              "long long": 8, "bool": 1, "float": 4, "double": 8}
+
+
+def _c_type(decl):
+    """('ptr' | 'void' | base type name) of one declaration without its parameter name."""
+    if "*" in decl:
+        return "ptr"
+    words = [w for w in decl.replace("const", " ").replace("restrict", " ").split()]
+    return " ".join(words)
+
+
+def prototypes():
+    out = {}
+    for path in _PROTO_HEADERS:
+        text = re.sub(r"/\*.*?\*/", "", open(path).read(), flags=re.S)
+        text = re.sub(r"^\s*#.*$", "", text, flags=re.M)
+        for ret, name, args in re.findall(r"([A-Za-z_][\w \*]*?)\b([a-z_][a-z0-9_]*)\s*\(([^;{}()]*)\)\s*;", text):
+            args = args.strip()
+            params = []
+            if args and args != "void":
+                for a in args.split(","):
+                    a = a.strip()
+                    # drop the parameter name (the last identifier after the type)
+                    m = re.match(r"(.*?[\*\s])([A-Za-z_]\w*)$", a)
+                    params.append(_c_type(m.group(1) if m else a))
+            out[name] = (_c_type(ret.strip()), params)
+    return out
+
+
+def _called_names():
+    import ast
+    names = set()
+    for rel in ("c-permutation-avoiding-convolution_amd/__init__.py", "bench.py"):
+        tree = ast.parse(open(os.path.join(ROOT, rel)).read())
+        for node in ast.walk(tree):
+            if isinstance(node, ast.Attribute) and isinstance(node.value, ast.Name) and node.value.id in ("lib", "L"):
+                if node.attr.startswith("pacg_"):
+                    names.add(node.attr)
+    return names
+
+
+def _size_ok(ctype, want):
+    if want == "ptr":
+        return ctype is C.c_void_p or ctype is C.c_char_p or (isinstance(ctype, type) and issubclass(ctype, C._Pointer))
+    if want in ("float", "double"):
+        return ctype in ((C.c_float,) if want == "float" else (C.c_double,))
+    return C.sizeof(ctype) == _INT_SIZES[want] and ctype not in (C.c_float, C.c_double, C.c_void_p)
+
+
+def binding_problems(protos, names, getfn):
+    bad = []
+    for name in sorted(names):
+        assert name in protos, "no prototype for %s" % name
+        ret, params = protos[name]
+        fn = getfn(name)
+        if params and fn.argtypes is None:
+            bad.append("%s: no argtypes" % name)
+            continue
+        at = list(fn.argtypes or [])
+        if len(at) != len(params):
+            bad.append("%s: %d argtypes for %d parameters" % (name, len(at), len(params)))
+            continue
+        for i, (ct, want) in enumerate(zip(at, params)):
+            if not _size_ok(ct, want):
+                bad.append("%s: parameter %d is %s, binding %s" % (name, i, want, ct))
+        if ret == "ptr" and not _size_ok(fn.restype, "ptr"):
+            bad.append("%s: pointer result without a pointer restype" % name)
+        elif ret in _INT_SIZES and _INT_SIZES[ret] == 8 and (fn.restype is None or C.sizeof(fn.restype) != 8):
+            bad.append("%s: 64-bit result with restype %s" % (name, fn.restype))
+    return bad
+
+
+def test_ctypes_bindings_match_the_c_prototypes(pacg):
+    protos = prototypes()
+    called = _called_names()
+    wrapped = [n for n in protos if n in vars(pacg) and callable(vars(pacg)[n])]
+    assert {"pacg_plan_create", "pacg_profile_convolve", "pacg_batch_convolve"} <= called
+    assert "permutation_avoiding_convolution_3d" in wrapped and len(wrapped) >= 40
+    bad = binding_problems(protos, called | set(wrapped), lambda n: getattr(pacg.lib, n))
+    assert not bad, bad
+
+
+def test_prototype_check_catches_a_missing_binding(pacg):
+    """A fresh handle of the library has no prototypes: the check reports the
+    pointer-truncating calls it would make, and a wrong-width binding."""
+    protos = prototypes()
+    assert protos["pacg_profile_convolve"][1][0] == "ptr" and len(protos["pacg_profile_convolve"][1]) == 9
+    fresh = C.CDLL(pacg.LIB_PATH)
+    bad = binding_problems(protos, {"pacg_profile_convolve", "pacg_plan_size"}, lambda n: getattr(fresh, n))
+    assert any(b.startswith("pacg_profile_convolve: no argtypes") for b in bad)
+    assert any(b.startswith("pacg_plan_size: no argtypes") for b in bad)
+    fresh.pacg_plan_size.argtypes = [C.c_int]  # a pointer parameter bound as int
+    fresh.pacg_plan_size.restype = C.c_longlong
+    bad = binding_problems(protos, {"pacg_plan_size"}, lambda n: getattr(fresh, n))
+    assert bad == ["pacg_plan_size: parameter 0 is ptr, binding <class 'ctypes.c_int'>"]
