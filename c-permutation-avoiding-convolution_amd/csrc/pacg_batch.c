@@ -1,12 +1,15 @@
 /*
- * Batched uneven-length convolutions, host side: packing, work items, the
- * per-length schedule autotuner and HIP-graph capture of the single
- * persistent launch (kernel: pacg_batch.hip).
+ * Batched uneven-length convolutions, host side: packing, work items, their
+ * assignment to workgroups, the per-length schedule autotuner and HIP-graph
+ * capture of the single persistent launch (kernel: pacg_batch.hip).
  *
  * Pairs are packed back to back in input order (pair i at offset
  * sum_{j<i} 2^m_j of each of x.re, x.im, k.re, k.im, y.re, y.im).  The
- * kernel sees them sorted longest first and cut into items of up to
- * (1024 threads * points per thread) / n pairs of one length n.
+ * kernel sees them sorted longest first and cut into items of 1024 / TPL
+ * pairs of one length n (TPL threads per pair).  Items are dealt to the
+ * workgroups longest processing time first (each to the least loaded one),
+ * then each workgroup's items are ordered by shape, so that it meets runs
+ * of one shape and can load the next item of a run ahead.
  */
 #define _GNU_SOURCE
 #include <stdlib.h>
@@ -20,7 +23,7 @@
 #define LMIN 8
 #define LMAX 14
 #define BATCH_THREADS 1024
-#define ITEM_POINTS (1 << 14)
+#define ITEM_POINTS (1 << 14)  /* points of the largest item (1024 threads x 16) */
 
 struct pacg_batch {
   int count;
@@ -36,7 +39,9 @@ struct pacg_batch {
   /* device */
   uint32_t* d_conv_off;
   pacg_batch_item_t* d_items;
+  uint32_t* d_wg_first;  /* blocks + 1 entries */
   int n_items;
+  int n_wg;              /* workgroups of the current item lists */
   int cap_items;
   void* d_tw;
 };
@@ -60,22 +65,93 @@ int pacg_batch_num_variants(int log2n) {
   return c;
 }
 
-/* convolutions per item: the 1024 threads' worth of TPL-thread groups, or for
- * lines of 8192 points and more (the kernel's virtual-thread path) as many
- * lines as fill 2^14 points */
+/* convolutions per item: the 1024 threads' worth of TPL-thread groups */
 static int per_item(int m, int v) {
   const pacg_batch_variant_t* r = variant_row(m, v);
-  if (!r) return 1;
-  if ((1 << m) >= 8 * BATCH_THREADS) return (1 << m) < ITEM_POINTS ? ITEM_POINTS >> m : 1;
+  if (!r || r->tpl < 1) return 1;
   int g = BATCH_THREADS / r->tpl;
   return g < 1 ? 1 : g;
 }
 
-/* items for the lengths whose mask bit is set; uploads them */
+/* Relative cost of an item of length 2^m with TPL threads per pair: every
+ * thread of the workgroup runs the whole item (a ragged item computes its
+ * missing pairs too and stores nothing), so the cost is the item's points
+ * (1024 x points per thread) times a per-point cost that grows with the
+ * number of stages. */
+static double item_cost(int m, int tpl) { return (double)BATCH_THREADS * (double)((1 << m) / tpl) * (4.0 + m); }
+
+typedef struct {
+  double cost;
+  int idx;
+} cost_idx_t;
+
+static int by_cost_desc(const void* a, const void* b) {
+  const cost_idx_t* x = (const cost_idx_t*)a;
+  const cost_idx_t* y = (const cost_idx_t*)b;
+  if (x->cost != y->cost) return x->cost > y->cost ? -1 : 1;
+  return x->idx - y->idx;
+}
+
+/* Deal items (built longest first, one shape after another) to w workgroups:
+ * longest processing time first, each to the least loaded workgroup; then
+ * group them by workgroup, keeping the built order inside a group (so the
+ * items of one shape stay together).  out[] receives the items, wg_first[]
+ * the w + 1 group starts. */
+static int deal_items(const pacg_batch_item_t* items, int n, int w, pacg_batch_item_t* out, uint32_t* wg_first) {
+  cost_idx_t* ci = (cost_idx_t*)malloc((size_t)n * sizeof(cost_idx_t));
+  int* owner = (int*)malloc((size_t)n * sizeof(int));
+  double* load = (double*)calloc((size_t)w, sizeof(double));
+  if (!ci || !owner || !load) {
+    free(ci);
+    free(owner);
+    free(load);
+    return pacg_fail(PACG_ERR_NOMEM, "out of host memory");
+  }
+  for (int i = 0; i < n; ++i) {
+    const pacg_batch_variant_t* r = variant_row(items[i].log2n, items[i].variant);
+    ci[i].cost = item_cost(items[i].log2n, r ? r->tpl : 1);
+    ci[i].idx = i;
+  }
+  qsort(ci, (size_t)n, sizeof(cost_idx_t), by_cost_desc);
+  for (int k = 0; k < n; ++k) {
+    int best = 0;
+    for (int j = 1; j < w; ++j)
+      if (load[j] < load[best]) best = j;
+    load[best] += ci[k].cost;
+    owner[ci[k].idx] = best;
+  }
+  for (int j = 0; j <= w; ++j) wg_first[j] = 0;
+  for (int i = 0; i < n; ++i) wg_first[owner[i] + 1]++;
+  for (int j = 0; j < w; ++j) wg_first[j + 1] += wg_first[j];
+  /* counting sort by owner, stable in the built order */
+  uint32_t* at = (uint32_t*)malloc((size_t)w * sizeof(uint32_t));
+  if (!at) {
+    free(ci);
+    free(owner);
+    free(load);
+    return pacg_fail(PACG_ERR_NOMEM, "out of host memory");
+  }
+  memcpy(at, wg_first, (size_t)w * sizeof(uint32_t));
+  for (int i = 0; i < n; ++i) out[at[owner[i]]++] = items[i];
+  free(at);
+  free(ci);
+  free(owner);
+  free(load);
+  return PACG_OK;
+}
+
+/* items for the lengths whose mask bit is set, dealt to the workgroups; uploads them */
 static int build_items(pacg_batch* b, unsigned mask) {
   int n = 0;
   pacg_batch_item_t* items = (pacg_batch_item_t*)malloc((size_t)(b->count + 1) * sizeof(pacg_batch_item_t));
-  if (!items) return pacg_fail(PACG_ERR_NOMEM, "out of host memory");
+  pacg_batch_item_t* dealt = (pacg_batch_item_t*)malloc((size_t)(b->count + 1) * sizeof(pacg_batch_item_t));
+  uint32_t* wg_first = (uint32_t*)malloc((size_t)(b->blocks + 1) * sizeof(uint32_t));
+  if (!items || !dealt || !wg_first) {
+    free(items);
+    free(dealt);
+    free(wg_first);
+    return pacg_fail(PACG_ERR_NOMEM, "out of host memory");
+  }
   int pos = 0;
   while (pos < b->count) {
     int m = b->log2n[b->order[pos]];
@@ -93,12 +169,19 @@ static int build_items(pacg_batch* b, unsigned mask) {
     }
     pos = end;
   }
+  int w = n < b->blocks ? n : b->blocks;
   int rc = PACG_OK;
-  if (n > 0 && pacg_memcpy_h2d(b->d_items, items, (size_t)n * sizeof(pacg_batch_item_t), b->stream))
+  if (n > 0) rc = deal_items(items, n, w, dealt, wg_first);
+  if (!rc && n > 0 &&
+      (pacg_memcpy_h2d(b->d_items, dealt, (size_t)n * sizeof(pacg_batch_item_t), b->stream) ||
+       pacg_memcpy_h2d(b->d_wg_first, wg_first, (size_t)(w + 1) * sizeof(uint32_t), b->stream)))
     rc = pacg_fail(PACG_ERR_HIP, "item upload failed");
   if (!rc && pacg_stream_sync(b->stream)) rc = pacg_fail(PACG_ERR_HIP, "item upload failed");
   free(items);
-  b->n_items = n;
+  free(dealt);
+  free(wg_first);
+  b->n_items = rc ? 0 : n;
+  b->n_wg = rc ? 0 : w;
   return rc;
 }
 
@@ -118,6 +201,7 @@ int pacg_batch_destroy(pacg_batch* b) {
   if (b->graph) pacg_graph_destroy(b->graph);
   pacg_dev_free(b->d_conv_off);
   pacg_dev_free(b->d_items);
+  pacg_dev_free(b->d_wg_first);
   pacg_dev_free(b->d_tw);
   if (b->own_stream && b->stream) pacg_stream_destroy(b->stream);
   free(b->log2n);
@@ -152,9 +236,9 @@ int pacg_batch_create(pacg_batch** out, int count, const int* log2_len) {
     off += 1LL << m;
   }
   b->total = off;
-  if (off >= (1LL << 32)) {
+  if (off >= (1LL << 30)) { /* the kernel addresses each array with 32-bit byte offsets */
     pacg_batch_destroy(b);
-    return pacg_fail(PACG_ERR_SIZE, "batch of %lld points exceeds 2^32", off);
+    return pacg_fail(PACG_ERR_SIZE, "batch of %lld points reaches 2^30", off);
   }
   sort_longest_first(b->log2n, count, b->order);
   for (int m = 0; m <= LMAX; ++m) b->variant[m] = 0;
@@ -169,6 +253,7 @@ int pacg_batch_create(pacg_batch** out, int count, const int* log2_len) {
   b->cap_items = count + 1;
   if (pacg_dev_malloc((void**)&b->d_conv_off, (size_t)count * sizeof(uint32_t)) ||
       pacg_dev_malloc((void**)&b->d_items, (size_t)b->cap_items * sizeof(pacg_batch_item_t)) ||
+      pacg_dev_malloc((void**)&b->d_wg_first, (size_t)(b->blocks + 1) * sizeof(uint32_t)) ||
       pacg_dev_malloc(&b->d_tw, (size_t)ITEM_POINTS * 2 * sizeof(float))) {
     rc = pacg_fail(PACG_ERR_NOMEM, "device allocation failed");
     goto fail;
@@ -235,10 +320,11 @@ static int launch(pacg_batch* b, const float* xr, const float* xi, const float* 
   a.yi = yi;
   a.conv_off = b->d_conv_off;
   a.items = b->d_items;
+  a.wg_first = b->d_wg_first;
   a.n_items = (uint32_t)b->n_items;
   a.tw = b->d_tw;
-  int blocks = b->n_items < b->blocks ? b->n_items : b->blocks;
-  int rc = pacg_batch_launch(&a, blocks, b->stream);
+  a.total = (uint32_t)b->total;
+  int rc = pacg_batch_launch(&a, b->n_wg, b->stream);
   if (rc) return pacg_fail(PACG_ERR_HIP, "batched launch failed: %s", pacg_hip_error_string(rc));
   return PACG_OK;
 }

@@ -9,8 +9,8 @@
 extern "C" {
 #endif
 
-/* up to 2^14 / n convolutions of one length n, at sorted positions
- * [first, first + count) */
+/* up to 1024 / TPL convolutions of one length n (TPL threads each), at sorted
+ * positions [first, first + count) */
 typedef struct {
   uint32_t first;
   uint16_t count;
@@ -26,9 +26,11 @@ typedef struct {
   float* yr;
   float* yi;
   const uint32_t* conv_off;          /* element offset of the pair at sorted position i */
-  const pacg_batch_item_t* items;
+  const pacg_batch_item_t* items;    /* grouped by workgroup, each group ordered by shape */
+  const uint32_t* wg_first;          /* workgroup w runs items [wg_first[w], wg_first[w + 1]) */
   uint32_t n_items;
   const void* tw;                    /* w_{2^14}^p, p < 2^14, (re, im) float pairs */
+  uint32_t total;                    /* points in each array (< 2^30) */
 } pacg_batch_args_t;
 
 typedef struct {

@@ -6,16 +6,25 @@
 // entirely inside one workgroup: k and x are read once, y is written once,
 // the spectrum never leaves registers and no permutation is executed.
 //
-// Work is cut on the host into *items*: up to 2^14 / n convolutions of one
-// length n (2^14 points per item), sorted longest first.  A workgroup of
-// 1024 threads takes items blockIdx.x, blockIdx.x + gridDim.x, ... until none
-// are left (every wave reaches the same exit).  Within an item, the n / E
-// threads of one convolution each hold E points per register phase; the
-// phases exchange through LDS (at most 2^14 points, 128 KB of (re, im) float
-// pairs, one 8-byte access per point; slots swizzled by lds_swz).
-// Schedules per length come from a variant table the autotuner picks from.
+// Work is cut on the host into *items*: 1024 / TPL convolutions of one length
+// n, where the TPL = n / E threads of one convolution hold E (8 or 16) points
+// each per register phase.  The host deals the items to the workgroups
+// (longest processing time first) and orders each workgroup's list by shape,
+// so a workgroup meets a run of items of one shape: while it transforms the
+// k of one item it has the x of that item in flight, and during its conj(A)
+// phases the k of the next item of the run.  The loads are unconditional
+// (a missing convolution of a ragged item reads the item's first pair and
+// stores nothing), so they stay in flight until their first use.  Global
+// accesses go through buffer resources: a 32-bit per-thread byte offset plus
+// a compile-time offset per element (scalar), bounded by the array size.
+// Phases exchange points through LDS: (re, im) float pairs, one 8-byte access
+// per point, one pad slot per 16 points (per 8 for 8-point shapes), so a slot
+// is a per-thread base plus a compile-time offset; an item of 16384 points
+// uses one buffer (two barriers per exchange), one of 8192 points alternates
+// two.
 #include <hip/hip_runtime.h>
 
+#include <stddef.h>
 #include <stdint.h>
 
 #include "pacg_batch.h"
@@ -23,17 +32,82 @@
 
 namespace pacg {
 
+static_assert(sizeof(pacg_batch_item_t) == 8 && offsetof(pacg_batch_item_t, count) == 4 &&
+                  offsetof(pacg_batch_item_t, log2n) == 6 && offsetof(pacg_batch_item_t, variant) == 7,
+              "item record read as two dwords");
+
 constexpr int BT = 1024;              // threads per workgroup
-constexpr int BPTS = 1 << 14;         // points per item
-constexpr int BLD = BPTS;             // LDS slots of (re, im) pairs (swizzled, no padding)
+constexpr int BPTS = 1 << 14;         // points of the largest item (BT x 16)
+constexpr int BLD = BPTS + BPTS / 8;  // LDS slots of (re, im) pairs: two 8192-point buffers padded 1 in 8
+
+// LDS slot of point s of an exchange tile padded with one slot per 2^PS
+// points.  For a sum of two positions with disjoint bits (a per-thread part
+// and a compile-time part) the slot is the sum of their slots, so the
+// compile-time part lands in the instruction's offset field.  16 consecutive
+// lanes of an 8-byte access then meet 16 different slots modulo 16 in every
+// phase layout of the 16-point shapes (pad 1 in 16: runs of consecutive
+// points, or points 16 apart for the bottom phase, whose slots step by 17)
+// and of the 8-point shapes (pad 1 in 8: points 8 or 64 apart step by 9 or 72).
+template <int PS>
+__device__ __forceinline__ constexpr int lds_pad(int s) {
+  return s + (s >> PS);
+}
+
+using rsrc_t = __amdgpu_buffer_rsrc_t;
+// raw buffer resource over `bytes` bytes (out-of-range loads read 0, stores
+// drop); the last argument of the accesses is the cache policy, 2 = nt
+__device__ __forceinline__ rsrc_t make_rsrc(const void* p, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ float buf_load(rsrc_t r, uint32_t voff, uint32_t soff) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
+}
+__device__ __forceinline__ void buf_store(float v, rsrc_t r, uint32_t voff, uint32_t soff) {
+  // non-temporal: y is not read again by this launch (about 6 % faster than
+  // the default policy on the 4096-pair batch)
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, (int)voff, (int)soff, 2);
+}
+
+// Group index o of group c of thread tl in phase PH.  The top phase (the one
+// that meets global memory) deals groups out as tl + TPL * c, so the lanes of
+// a wave touch consecutive points; the other phases keep a thread's groups
+// adjacent, tl * (E / R) + c, so a thread's points stay in a block of E * tl
+// points and the exchanges below the top phase can stay inside a wavefront.
+template <class SH, int PH>
+__device__ __forceinline__ int group_of(int tl, int c) {
+  constexpr int lgg = SH::lgE() - SH::lgR(PH);  // log2 groups per thread
+  if constexpr (PH == SH::NP - 1)
+    return tl + SH::TPL * c;
+  else
+    return (tl << lgg) | c;
+}
+// Position (within the line) of element (c, r) of thread tl in phase PH: the
+// phase's R points of group o are spaced by its lower span 2^lgl.  The bits of
+// tl, c and r land on disjoint bits, so bpos(tl, c, r) = bpos(tl, 0, 0) +
+// bpos(0, c, r).
+template <class SH, int PH>
+__device__ __forceinline__ int bpos(int tl, int c, int r) {
+  constexpr int lgl = SH::lglam(SH::plo(PH));
+  constexpr int lgR = SH::lgR(PH);
+  const int o = group_of<SH, PH>(tl, c);
+  return ((o >> lgl) << (lgl + lgR)) | (r << lgl) | (o & ((1 << lgl) - 1));
+}
+
+// the local index r of phase PH whose mixed digit reversal is k
+template <class SH, int PH>
+constexpr int rev_index(int k) {
+  for (int r = 0; r < (1 << SH::lgR(PH)); ++r)
+    if (SH::rev_phase(PH, r) == k) return r;
+  return -1;
+}
 
 // Phase twiddle of phase PH.  The R points of a butterfly group take
 // w^(mu * kr), kr = 0 .. R-1, with w = w_P^(2^sh); only the base b = w^mu
 // comes from the two LDS tables of w_{2^14} (index e = (mu << sh) << (14 -
 // log2 P); when the phase's stages span at most 128 points, e is a multiple
 // of 128 and b = hi[e >> 7] exactly).  The other powers follow as products
-// b^k = b^(k/2) * b^(k - k/2), at most three products deep: 6 complex
-// multiplies instead of 7 table reads and products for a radix-8 group.
+// b^k = b^(k/2) * b^(k - k/2), at most four products deep: R - 2 complex
+// multiplies instead of R - 1 table reads and products.
 template <class SH, int PH, bool CJ>
 __device__ __forceinline__ void batch_phase_twiddle(float* vr, float* vi, int tl, const float2* twh,
                                                     const float2* twl_) {
@@ -44,10 +118,10 @@ __device__ __forceinline__ void batch_phase_twiddle(float* vr, float* vi, int tl
     constexpr int lgl = SH::lglam(lo);
     constexpr int sh = SH::LGP - SH::lglam(SH::phi(PH) + 1);
     constexpr bool ONE_TABLE = SH::lglam(SH::phi(PH) + 1) <= 7;
-    static_assert(R <= 8, "phase radix");
+    static_assert(R <= 16, "phase radix");
     sfor<SH::E / R>([&](auto c_) PACG_INL {
       constexpr int c = decltype(c_)::value;
-      const int mu = (tl + SH::TPL * c) & ((1 << lgl) - 1);
+      const int mu = group_of<SH, PH>(tl, c) & ((1 << lgl) - 1);
       const int e = (mu << sh) << TW_SHIFT;
       float pr[R], pi[R];
       if constexpr (ONE_TABLE) {
@@ -61,95 +135,159 @@ __device__ __forceinline__ void batch_phase_twiddle(float* vr, float* vi, int tl
         pi[1] = h.x * l.y + h.y * l.x;
       }
       if constexpr (CJ) pi[1] = -pi[1];
-      sfor<R - 2>([&](auto k_) PACG_INL {
-        constexpr int k = 2 + decltype(k_)::value;
-        constexpr int a = k / 2, b = k - k / 2;
-        pr[k] = pr[a] * pr[b] - pi[a] * pi[b];
-        pi[k] = pr[a] * pi[b] + pi[a] * pr[b];
-      });
-      sfor<R>([&](auto r_) PACG_INL {
-        constexpr int r = decltype(r_)::value;
-        constexpr int kr = SH::rev_phase(PH, r);
-        if constexpr (kr != 0) cmul<float>(vr[c * R + r], vi[c * R + r], pr[kr], pi[kr]);
+      // each power is applied as soon as it exists; only powers up to R / 2
+      // feed later products, so at most R / 2 stay live
+      sfor<R - 1>([&](auto k_) PACG_INL {
+        constexpr int k = 1 + decltype(k_)::value;
+        if constexpr (k >= 2) {
+          constexpr int a = k / 2, b = k - k / 2;
+          pr[k] = pr[a] * pr[b] - pi[a] * pi[b];
+          pi[k] = pr[a] * pi[b] + pi[a] * pr[b];
+        }
+        constexpr int r = rev_index<SH, PH>(k);
+        cmul<float>(vr[c * R + r], vi[c * R + r], pr[k], pi[k]);
       });
     });
   }
 }
 
+// Thread (within its convolution) that holds point s of a line in a phase
+// whose lower stages span 2^lgl points, of radix 2^lgr, with 2^lgg groups per
+// thread (group_of: the top phase deals groups out, the others keep them
+// adjacent).
+constexpr int holder(int s, int lgl, int lgr, int lgg, bool top, int tpl) {
+  const int o = ((s >> (lgl + lgr)) << lgl) | (s & ((1 << lgl) - 1));
+  return top ? o % tpl : o >> lgg;
+}
+// An exchange between phases A and B stays inside one wavefront when every
+// point is held by threads of the same wave in both layouts (always when a
+// convolution's threads fit in one wave).
+template <class SH, int A, int B>
+constexpr bool wave_local() {
+  if (SH::TPL <= 64) return true;
+  const int la = SH::lglam(SH::plo(A)), ra = SH::lgR(A), ga = SH::lgE() - ra;
+  const int lb = SH::lglam(SH::plo(B)), rb = SH::lgR(B), gb = SH::lgE() - rb;
+  const bool ta = A == SH::NP - 1, tb = B == SH::NP - 1;
+  for (int s = 0; s < SH::P; ++s)
+    if (holder(s, la, ra, ga, ta, SH::TPL) / 64 != holder(s, lb, rb, gb, tb, SH::TPL) / 64) return false;
+  return true;
+}
+
+// A run of items of one shape SH starting at item i (the workgroup's list
+// ends at `end`); returns the index of the first item not of this shape.
 template <class SH>
-__device__ __forceinline__ void batch_item(const pacg_batch_args_t& a, const pacg_batch_item_t it, float2* lds,
-                                           const float2* twh, const float2* twl_) {
+__device__ __forceinline__ uint32_t batch_run(const pacg_batch_args_t& a, uint32_t i, const uint32_t end,
+                                              float2* lds, const float2* twh, const float2* twl_) {
   constexpr int P = SH::P, E = SH::E, TPL = SH::TPL, NP = SH::NP;
-  static_assert(BT % TPL == 0, "threads per convolution (BT / TPL convolutions per item)");
-  // Recompute every per-thread index inside the persistent loop: hoisted out
-  // of it, the positions and LDS addresses of all phases would stay live.
-  int tid = threadIdx.x;
-  asm volatile("" : "+v"(tid));
-  const int g = tid / TPL;
-  const int tl = tid % TPL;
-  const bool valid = g < (int)it.count;
-  const uint32_t base = valid ? a.conv_off[it.first + g] : 0u;
-  // phase twiddles: w_P^idx = w_{2^14}^e from two LDS tables (batch_phase_twiddle)
-  // LDS slot of (c, r) in phase PH: convolution g at g * P, swizzled (lds_swz)
-  auto li = [&](auto PH_, int c, int r) PACG_INL {
-    constexpr int PH = decltype(PH_)::value;
-    return ((g * P) ^ lds_swz(pos<SH, PH>(tl, 0, 0))) ^ lds_swz(pos<SH, PH>(0, c, r));
+  constexpr int G = BT / TPL;                // convolutions per item
+  constexpr int IPTS = BT * E;               // points per item
+  constexpr bool DBUF = IPTS <= BPTS / 2;    // two exchange buffers fit
+  constexpr int PS = E == 16 ? 4 : 3;        // one pad slot per 2^PS points
+  constexpr int BUFSZ = IPTS + (IPTS >> PS);
+  static_assert(BT % TPL == 0 && (E == 8 || E == 16) && IPTS <= BPTS && (DBUF ? 2 : 1) * BUFSZ <= BLD,
+                "item shape");
+  // Per-thread indexes are re-derived (through an opaque copy of the thread
+  // index) at every transform and exchange: derived once, every LDS slot and
+  // global position of every phase would be loop-invariant, hoisted out of
+  // the item loop and kept live, and the 16-point shapes would spill.
+  int g, tl;
+  auto fresh = [&]() PACG_INL {
+    int t = threadIdx.x;
+    asm volatile("" : "+v"(t));
+    g = t / TPL;
+    tl = t % TPL;
   };
+  fresh();
+  const pacg_batch_item_t it0 = a.items[i];
+  const uint32_t key = ((uint32_t)it0.log2n << 8) | it0.variant;
+  const float scale = 1.0f / (float)P;
+  const uint32_t bytes = a.total * 4u;
+  const rsrc_t rkr = make_rsrc(a.kr, bytes), rki = make_rsrc(a.ki, bytes);
+  const rsrc_t rxr = make_rsrc(a.xr, bytes), rxi = make_rsrc(a.xi, bytes);
+  const rsrc_t ryr = make_rsrc(a.yr, bytes), ryi = make_rsrc(a.yi, bytes);
 
-  // Spectrum A^T k and the read-ahead of x: in registers (at most 8 points per
-  // thread below 2^13 points, so both fit beside the working points).
-  static_assert(E <= 8 && P <= 4096, "register spectrum and x read-ahead");
-  float vr[E], vi[E];  // working points
-  float pr[E], pi[E];  // x, loaded while k is transformed
-  float sr[E], si[E];  // the spectrum
-
-  auto load_top = [&](const float* __restrict__ br, const float* __restrict__ bi, float* dr, float* di) PACG_INL {
+  // element offset of this thread's convolution in the item: its own pair, or
+  // the item's first pair where a ragged item has fewer than G
+  auto pair_base = [&](const pacg_batch_item_t& it) PACG_INL -> uint32_t {
+    return a.conv_off[it.first + ((uint32_t)g < (uint32_t)it.count ? (uint32_t)g : 0u)];
+  };
+  // the points of the top (first DIF) phase from global memory, part Q of NQ
+  // (the read-ahead is spread over the phases of a transform, so that the
+  // workgroups do not all meet one burst of loads at each item boundary):
+  // position pos(tl, 0, 0) + pos(0, c, r), the second part a constant
+  auto load_top = [&](auto Q_, auto NQ_, rsrc_t br, rsrc_t bi, uint32_t b, float* dr, float* di) PACG_INL {
     constexpr int PH = NP - 1;
     constexpr int R = 1 << SH::lgR(PH);
-    sfor<E / R>([&](auto c_) PACG_INL {
-      constexpr int c = decltype(c_)::value;
-      sfor<R>([&](auto r_) PACG_INL {
-        constexpr int r = decltype(r_)::value;
-        const int s = pos<SH, PH>(tl, c, r);
-        dr[c * R + r] = valid ? br[base + s] : 0.f;
-        di[c * R + r] = valid ? bi[base + s] : 0.f;
-      });
+    constexpr int Q = decltype(Q_)::value, NQ = decltype(NQ_)::value;
+    constexpr int E0 = Q * E / NQ, E1 = (Q + 1) * E / NQ;
+    const uint32_t vo = (b + (uint32_t)bpos<SH, PH>(tl, 0, 0)) * 4u;
+    sfor<E1 - E0>([&](auto j_) PACG_INL {
+      constexpr int e = E0 + decltype(j_)::value;
+      constexpr int c = e / R, r = e % R;
+      const uint32_t so = (uint32_t)bpos<SH, PH>(0, c, r) * 4u;  // folds to a constant
+      dr[e] = buf_load(br, vo, so);
+      di[e] = buf_load(bi, vo, so);
     });
   };
-  // Exchange number X of the item goes through LDS buffer X % 2 (an item holds
-  // at most 1024 * 8 points, so two buffers fit): the buffer it writes was last
-  // read two exchanges ago, before the barrier of the exchange in between, so
-  // one barrier per exchange suffices.
+  using One = std::integral_constant<int, 1>;
+  using Zero = std::integral_constant<int, 0>;
+  // the read-ahead of a transform is issued in NQ parts at its phases 0 ..
+  // NQ - 1: the last part still has a whole phase to arrive
+  constexpr int NQ = NP > 1 ? NP - 1 : 1;
+  using NQc = std::integral_constant<int, NQ>;
+
+  float vr[E], vi[E];  // working points
+  // exchange number X of the item: through LDS buffer X % 2 when two fit (the
+  // buffer it writes was last read two exchanges ago, before the barrier of
+  // the one in between: one barrier), else through the one buffer (two).
+  // With one buffer, an exchange that stays inside each wavefront needs no
+  // workgroup barrier: the slots a wave writes were last read by that same
+  // wave (in the layout this exchange starts from), and its own LDS accesses
+  // complete in order once it waits for its writes.
   auto exchange = [&](auto FROM_, auto TO_, auto X_) PACG_INL {
     constexpr int FROM = decltype(FROM_)::value;
     constexpr int TO = decltype(TO_)::value;
     constexpr int RF = 1 << SH::lgR(FROM);
     constexpr int RT = 1 << SH::lgR(TO);
-    constexpr int BUF = (decltype(X_)::value & 1) * (BPTS / 2);
-    static_assert(BT * E <= BPTS / 2, "two exchange buffers per item");
-    sfor<E / RF>([&](auto c_) PACG_INL {
-      constexpr int c = decltype(c_)::value;
-      sfor<RF>([&](auto r_) PACG_INL {
-        constexpr int r = decltype(r_)::value;
-        const int l = BUF + li(FROM_, c, r);
-        lds[l] = make_float2(vr[c * RF + r], vi[c * RF + r]);
+    constexpr int BUF = DBUF ? (decltype(X_)::value & 1) * BUFSZ : 0;
+    constexpr bool LOCAL = !DBUF && wave_local<SH, FROM, TO>();
+    if constexpr (!DBUF && !LOCAL) __syncthreads();
+    fresh();
+    {
+      float2* w = lds + BUF + lds_pad<PS>(g * P + bpos<SH, FROM>(tl, 0, 0));
+      sfor<E / RF>([&](auto c_) PACG_INL {
+        constexpr int c = decltype(c_)::value;
+        sfor<RF>([&](auto r_) PACG_INL {
+          constexpr int r = decltype(r_)::value;
+          w[lds_pad<PS>(bpos<SH, FROM>(0, c, r))] = make_float2(vr[c * RF + r], vi[c * RF + r]);
+        });
       });
-    });
-    __syncthreads();
-    sfor<E / RT>([&](auto c_) PACG_INL {
-      constexpr int c = decltype(c_)::value;
-      sfor<RT>([&](auto r_) PACG_INL {
-        constexpr int r = decltype(r_)::value;
-        const int l = BUF + li(TO_, c, r);
-        const float2 w = lds[l];
-        vr[c * RT + r] = w.x;
-        vi[c * RT + r] = w.y;
+    }
+    if constexpr (LOCAL)
+      asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");  // this wave's writes done; no reordering
+    else
+      __syncthreads();
+    fresh();
+    {
+      const float2* w = lds + BUF + lds_pad<PS>(g * P + bpos<SH, TO>(tl, 0, 0));
+      sfor<E / RT>([&](auto c_) PACG_INL {
+        constexpr int c = decltype(c_)::value;
+        sfor<RT>([&](auto r_) PACG_INL {
+          constexpr int r = decltype(r_)::value;
+          const float2 v = w[lds_pad<PS>(bpos<SH, TO>(0, c, r))];
+          vr[c * RT + r] = v.x;
+          vi[c * RT + r] = v.y;
+        });
       });
-    });
+    }
   };
-  auto dif = [&](auto X0_) PACG_INL {  // X0: number of the first exchange
+  // A^T of the working points; X0: number of the first exchange; hook(i) runs
+  // at the start of the i-th phase
+  auto dif = [&](auto X0_, auto&& hook) PACG_INL {
+    fresh();
     sfor<NP>([&](auto i_) PACG_INL {
       constexpr int PH = NP - 1 - decltype(i_)::value;
+      hook(i_);
       if constexpr (PH < NP - 1)
         exchange(std::integral_constant<int, PH + 1>{}, std::integral_constant<int, PH>{},
                  std::integral_constant<int, decltype(X0_)::value + NP - 2 - PH>{});
@@ -158,211 +296,116 @@ __device__ __forceinline__ void batch_item(const pacg_batch_args_t& a, const pac
     });
   };
 
-  load_top(a.kr, a.ki, vr, vi);
-  load_top(a.xr, a.xi, pr, pi);  // x arrives while k is transformed
-  dif(std::integral_constant<int, 0>{});  // A^T k
+  float kr[E], ki[E];  // k of the current item, loaded ahead
+  pacg_batch_item_t it = it0;
+  uint32_t base = pair_base(it);
+  load_top(Zero{}, One{}, rkr, rki, base, kr, ki);
+  for (;;) {
+    // The next item's record is read now and its pair offset after A^T k:
+    // each has a whole transform to arrive before it is used.
+    const uint32_t nx = i + 1;
+    const bool has_next = nx < end;
+    uint2 nrec = make_uint2(0u, 0u);
+    if (has_next) nrec = reinterpret_cast<const uint2*>(a.items)[nx];
+    fresh();
+    float xr[E], xi[E];  // x arrives while k is transformed, a part per phase
 #pragma unroll
-  for (int e = 0; e < E; ++e) {
-    sr[e] = vr[e];
-    si[e] = vi[e];
-    vr[e] = pr[e];
-    vi[e] = pi[e];
-  }
-  dif(std::integral_constant<int, NP - 1>{});  // A^T x
-  const float scale = 1.0f / (float)P;
+    for (int e = 0; e < E; ++e) {
+      vr[e] = kr[e];
+      vi[e] = ki[e];
+    }
+    dif(Zero{}, [&](auto i_) PACG_INL {
+      if constexpr (decltype(i_)::value < NQ) load_top(i_, NQc{}, rxr, rxi, base, xr, xi);
+    });  // A^T k
+    // the record is the same in every lane: make it scalar, so that the
+    // branches on it and the exit of this loop are uniform
+    pacg_batch_item_t itn;
+    {
+      const uint32_t lo = __builtin_amdgcn_readfirstlane(nrec.x), hi = __builtin_amdgcn_readfirstlane(nrec.y);
+      itn.first = lo;
+      itn.count = (uint16_t)(hi & 0xFFFFu);
+      itn.log2n = (uint8_t)((hi >> 16) & 0xFFu);
+      itn.variant = (uint8_t)(hi >> 24);
+    }
+    const bool more = has_next && (((uint32_t)itn.log2n << 8) | itn.variant) == key;
+    uint32_t nbase = base;
+    if (more) nbase = pair_base(itn);
+    float sr[E], si[E];
 #pragma unroll
-  for (int e = 0; e < E; ++e) {
-    cmul<float>(vr[e], vi[e], sr[e], si[e]);
-    vr[e] *= scale;
-    vi[e] *= scale;
-  }
-  sfor<NP>([&](auto i_) PACG_INL {
-    constexpr int PH = decltype(i_)::value;
-    if constexpr (PH > 0)
-      exchange(std::integral_constant<int, PH - 1>{}, std::integral_constant<int, PH>{},
-               std::integral_constant<int, 2 * (NP - 1) + PH - 1>{});
-    batch_phase_twiddle<SH, PH, true>(vr, vi, tl, twh, twl_);
-    phase_stages<float, SH, PH, M_DITC>(vr, vi);
-  });
-  {
-    constexpr int PH = NP - 1;
-    constexpr int R = 1 << SH::lgR(PH);
-    uint32_t sb = base;
-    asm volatile("" : "+v"(sb));  // keep the store addresses from being formed early
-    sfor<E / R>([&](auto c_) PACG_INL {
-      constexpr int c = decltype(c_)::value;
-      sfor<R>([&](auto r_) PACG_INL {
-        constexpr int r = decltype(r_)::value;
-        const uint32_t s = sb + (uint32_t)pos<SH, PH>(tl, c, r);
-        if (valid) {
-          a.yr[s] = vr[c * R + r];
-          a.yi[s] = vi[c * R + r];
-        }
-      });
-    });
-  }
-  __syncthreads();  // LDS is reused by the next item
-}
-
-// Lines of BT * 8 points and more (2^13, 2^14): an item holds G = 2^14 / P
-// convolutions (2 at 2^13, 1 at 2^14) and each thread runs VT = G * TPL / BT
-// virtual threads of 8 points in turn (virtual thread vtl: convolution
-// vtl / TPL, thread vtl % TPL of it).  The lines stay in LDS between phases
-// (the same LDS traffic as the exchange model, half the live registers).  The
-// spectrum A^T k leaves the last forward phase of k in the registers of the
-// thread that later holds the same positions of x (phase 0 of virtual thread
-// v, for both), so it stays there: 2 x VT x 8 values, no global round trip.
-template <class SH>
-__device__ __forceinline__ void batch_item_vt(const pacg_batch_args_t& a, const pacg_batch_item_t it, float2* lds,
-                                              const float2* twh, const float2* twl_) {
-  constexpr int P = SH::P, E = SH::E, NP = SH::NP, TPL = SH::TPL;
-  constexpr int G = BPTS / P;
-  constexpr int VT = G * TPL / BT;
-  static_assert(VT * BT == G * TPL && TPL % BT == 0 && E <= 8 && (G == 1 || G == 2), "virtual threads");
-  int tid = threadIdx.x;
-  asm volatile("" : "+v"(tid));
-  bool valid_g[G];
-  uint32_t base_g[G];
+    for (int e = 0; e < E; ++e) {
+      sr[e] = vr[e] * scale;
+      si[e] = vi[e] * scale;
+      vr[e] = xr[e];
+      vi[e] = xi[e];
+    }
+    dif(std::integral_constant<int, NP - 1>{}, [&](auto) PACG_INL {});  // A^T x
 #pragma unroll
-  for (int g = 0; g < G; ++g) {
-    valid_g[g] = g < (int)it.count;
-    base_g[g] = valid_g[g] ? a.conv_off[it.first + g] : 0u;
-  }
-  const float scale = 1.0f / (float)P;
-  float sr[VT][E], si[VT][E];  // the spectrum at the phase-0 positions of each virtual thread
-
-  // one phase for one virtual thread: read (global or LDS), compute, write back;
-  // V = the virtual thread index when the phase touches the spectrum (tails 1, 2)
-  auto run = [&](auto PH_, auto MODE_, auto FROMG_, auto TAIL_, auto V_, int vtl, const float* __restrict__ gr,
-                 const float* __restrict__ gi) PACG_INL {
-    constexpr int V = decltype(V_)::value;
-    constexpr int PH = decltype(PH_)::value;
-    constexpr int MODE = decltype(MODE_)::value;
-    constexpr bool from_global = decltype(FROMG_)::value;
-    constexpr int tail = decltype(TAIL_)::value;
-    constexpr int R = 1 << SH::lgR(PH);
-    // TPL >= BT: the convolution index is uniform over the workgroup
-    const int g = G == 1 ? 0 : vtl / TPL;
-    const int tl = G == 1 ? vtl : vtl % TPL;
-    const bool valid = G == 1 ? valid_g[0] : (g ? valid_g[G - 1] : valid_g[0]);
-    const uint32_t base = G == 1 ? base_g[0] : (g ? base_g[G - 1] : base_g[0]);
-    const int lg = g * P;  // this convolution's LDS region
-    float vr[E], vi[E];
-    sfor<E / R>([&](auto c_) PACG_INL {
-      constexpr int c = decltype(c_)::value;
-      sfor<R>([&](auto r_) PACG_INL {
-        constexpr int r = decltype(r_)::value;
-        const int sp = pos<SH, PH>(tl, c, r);
-        if constexpr (from_global) {
-          vr[c * R + r] = valid ? gr[base + sp] : 0.f;
-          vi[c * R + r] = valid ? gi[base + sp] : 0.f;
-        } else {
-          const int l = lg + (lds_swz(pos<SH, PH>(tl, 0, 0)) ^ lds_swz(pos<SH, PH>(0, c, r)));
-          const float2 w = lds[l];
-          vr[c * R + r] = w.x;
-          vi[c * R + r] = w.y;
-        }
-      });
-    });
-    if constexpr (MODE == M_DIF) {
-      phase_stages<float, SH, PH, M_DIF>(vr, vi);
-      batch_phase_twiddle<SH, PH, false>(vr, vi, tl, twh, twl_);
-    } else {
+    for (int e = 0; e < E; ++e) cmul<float>(vr[e], vi[e], sr[e], si[e]);
+    // conj(A); the next item's k arrives during it, a part per phase
+    fresh();
+    sfor<NP>([&](auto i_) PACG_INL {
+      constexpr int PH = decltype(i_)::value;
+      if constexpr (PH < NQ) {
+        if (more) load_top(i_, NQc{}, rkr, rki, nbase, kr, ki);
+      }
+      if constexpr (PH > 0)
+        exchange(std::integral_constant<int, PH - 1>{}, std::integral_constant<int, PH>{},
+                 std::integral_constant<int, 2 * (NP - 1) + PH - 1>{});
       batch_phase_twiddle<SH, PH, true>(vr, vi, tl, twh, twl_);
       phase_stages<float, SH, PH, M_DITC>(vr, vi);
-    }
-    // tail: 0 write LDS, 1 k spectrum -> sr/si, 2 x: product + conj(A) phase 0 -> LDS, 3 -> y
-    sfor<E / R>([&](auto c_) PACG_INL {
-      constexpr int c = decltype(c_)::value;
-      if constexpr (tail == 2) {
+    });
+    {
+      constexpr int PH = NP - 1;
+      constexpr int R = 1 << SH::lgR(PH);
+      fresh();
+      const bool valid = (uint32_t)g < (uint32_t)it.count;
+      const uint32_t vo = (base + (uint32_t)bpos<SH, PH>(tl, 0, 0)) * 4u;
+      sfor<E / R>([&](auto c_) PACG_INL {
+        constexpr int c = decltype(c_)::value;
         sfor<R>([&](auto r_) PACG_INL {
           constexpr int r = decltype(r_)::value;
-          cmul<float>(vr[c * R + r], vi[c * R + r], sr[V][c * R + r], si[V][c * R + r]);
-          vr[c * R + r] *= scale;
-          vi[c * R + r] *= scale;
-        });
-      }
-    });
-    if constexpr (PH == 0) {
-      if constexpr (tail == 2) phase_stages<float, SH, 0, M_DITC>(vr, vi);
-    }
-    sfor<E / R>([&](auto c_) PACG_INL {
-      constexpr int c = decltype(c_)::value;
-      sfor<R>([&](auto r_) PACG_INL {
-        constexpr int r = decltype(r_)::value;
-        const int sp = pos<SH, PH>(tl, c, r);
-        if constexpr (tail == 1) {
-          sr[V][c * R + r] = vr[c * R + r];
-          si[V][c * R + r] = vi[c * R + r];
-        } else if constexpr (tail == 3) {
+          const uint32_t so = (uint32_t)bpos<SH, PH>(0, c, r) * 4u;  // folds to a constant
           if (valid) {
-            a.yr[base + sp] = vr[c * R + r];
-            a.yi[base + sp] = vi[c * R + r];
+            buf_store(vr[c * R + r], ryr, vo, so);
+            buf_store(vi[c * R + r], ryi, vo, so);
           }
-        } else {
-          const int l = lg + (lds_swz(pos<SH, PH>(tl, 0, 0)) ^ lds_swz(pos<SH, PH>(0, c, r)));
-          lds[l] = make_float2(vr[c * R + r], vi[c * R + r]);
-        }
-      });
-    });
-  };
-
-  // A^T k (the spectrum into sr/si), then A^T x, product, conj(A).  In the
-  // forward phases the virtual threads run one after the other (one set of
-  // points live; unrolled, they spill), except in phase 0, where each keeps its
-  // own spectrum registers.  The conj(A) phases, with the spectrum gone, are
-  // unrolled so that the LDS reads of one virtual thread overlap the other's
-  // arithmetic.
-  sfor<2>([&](auto pass_) PACG_INL {
-    constexpr int pass = decltype(pass_)::value;
-    const float* gr = pass == 0 ? a.kr : a.xr;
-    const float* gi = pass == 0 ? a.ki : a.xi;
-    sfor<NP>([&](auto i_) PACG_INL {
-      constexpr int PH = NP - 1 - decltype(i_)::value;
-      constexpr int tail = PH > 0 ? 0 : (pass == 0 ? 1 : 2);
-      __syncthreads();
-      if constexpr (tail == 0) {
-#pragma nounroll
-        for (int v = 0; v < VT; ++v)
-          run(std::integral_constant<int, PH>{}, std::integral_constant<int, M_DIF>{},
-              std::integral_constant<bool, PH == NP - 1>{}, std::integral_constant<int, tail>{},
-              std::integral_constant<int, 0>{}, tid + v * BT, gr, gi);
-      } else {
-        sfor<VT>([&](auto v_) PACG_INL {
-          constexpr int v = decltype(v_)::value;
-          run(std::integral_constant<int, PH>{}, std::integral_constant<int, M_DIF>{},
-              std::integral_constant<bool, PH == NP - 1>{}, std::integral_constant<int, tail>{},
-              std::integral_constant<int, v>{}, tid + v * BT, gr, gi);
         });
-      }
-    });
-  });
-  sfor<NP - 1>([&](auto i_) PACG_INL {
-    constexpr int PH = 1 + decltype(i_)::value;
-    constexpr int tail = PH == NP - 1 ? 3 : 0;
-    __syncthreads();
-#pragma unroll
-    for (int v = 0; v < VT; ++v)
-      run(std::integral_constant<int, PH>{}, std::integral_constant<int, M_DITC>{}, std::integral_constant<bool, false>{},
-          std::integral_constant<int, tail>{}, std::integral_constant<int, 0>{}, tid + v * BT, a.xr, a.xi);
-  });
-  __syncthreads();  // LDS is reused by the next item
+      });
+    }
+    __syncthreads();  // LDS is reused by the next item
+    if (!more) return nx;
+    i = nx;
+    it = itn;
+    base = nbase;
+  }
 }
 
-// Variant table: (log2 n, variant) -> shape.  Stage 0 first; phases listed
-// by their top stage.  Kept in step with pacg_batch_variants[] below.
+// Variant table: (log2 n, variant) -> shape (stage count, 2-bit log radices
+// with stage 0 first, phase count, 4-bit top stage of each phase with phase 0
+// first).  Variant 0 of every length holds 16 points per thread (phases of
+// radix 4 x 4, one phase fewer than 8-point phases: fewer exchanges and
+// fewer phase twiddles); at 2^13 and 2^14 its phases below the top span 1024
+// points, one wavefront's share, so only the exchange with the top phase
+// needs the workgroup barrier.  The other variants are alternatives for the
+// autotuner (8 points per thread where listed).  Kept in step with
+// pacg_batch_variants[] below.
 #define PACG_BATCH_VARIANTS(X)                                          \
-  X(8, 0, 3, 0x2Fu, 3, 0x210u)        /* 8 | 8 | 4             */     \
-  X(8, 1, 4, 0xAAu, 4, 0x3210u)       /* 4 | 4 | 4 | 4         */     \
-  X(9, 0, 3, 0x3Fu, 3, 0x210u)        /* 8 | 8 | 8             */     \
-  X(9, 1, 4, 0xABu, 4, 0x3210u)       /* 8 | 4 | 4 | 4         */     \
-  X(10, 0, 4, 0x7Fu, 4, 0x3210u)      /* 8 | 8 | 8 | 2         */     \
-  X(10, 1, 4, 0xAFu, 4, 0x3210u)      /* 8 | 8 | 4 | 4         */     \
-  X(11, 0, 4, 0xBFu, 4, 0x3210u)      /* 8 | 8 | 8 | 4         */     \
-  X(11, 1, 5, 0x2F9u, 4, 0x4321u)     /* 2 4 | 8 | 8 | 4       */     \
-  X(12, 0, 4, 0xFFu, 4, 0x3210u)      /* 8 | 8 | 8 | 8         */     \
-  X(13, 0, 5, 0x1FFu, 5, 0x43210u)    /* 8 | 8 | 8 | 8 | 2     */     \
-  X(14, 0, 5, 0x3FEu, 5, 0x43210u)    /* 4 | 8 | 8 | 8 | 8 (two 8-point virtual threads) */
+  X(8, 0, 4, 0xAAu, 2, 0x31u)         /* 4 4 | 4 4                */  \
+  X(8, 1, 3, 0x2Fu, 3, 0x210u)        /* 8 | 8 | 4                */  \
+  X(9, 0, 5, 0x1AAu, 3, 0x431u)       /* 4 4 | 4 4 | 2            */  \
+  X(9, 1, 3, 0x3Fu, 3, 0x210u)        /* 8 | 8 | 8                */  \
+  X(10, 0, 5, 0x2AAu, 3, 0x431u)      /* 4 4 | 4 4 | 4            */  \
+  X(10, 1, 4, 0xAFu, 4, 0x3210u)      /* 8 | 8 | 4 | 4            */  \
+  X(11, 0, 5, 0x3AAu, 3, 0x431u)      /* 4 4 | 4 4 | 8            */  \
+  X(11, 1, 4, 0xBFu, 4, 0x3210u)      /* 8 | 8 | 8 | 4            */  \
+  X(12, 0, 6, 0xAAAu, 3, 0x531u)      /* 4 4 | 4 4 | 4 4          */  \
+  X(12, 1, 4, 0xFFu, 4, 0x3210u)      /* 8 | 8 | 8 | 8            */  \
+  X(13, 0, 6, 0xEAAu, 4, 0x5431u)     /* 4 4 | 4 4 | 4 | 8        */  \
+  X(13, 1, 7, 0x1AAAu, 4, 0x6531u)    /* 4 4 | 4 4 | 4 4 | 2      */  \
+  X(13, 2, 5, 0x1FFu, 5, 0x43210u)    /* 8 | 8 | 8 | 8 | 2        */  \
+  X(14, 0, 7, 0x2AAAu, 4, 0x6431u)    /* 4 4 | 4 4 | 4 | 4 4      */  \
+  X(14, 1, 7, 0x2AAAu, 4, 0x6531u)    /* 4 4 | 4 4 | 4 4 | 4      */  \
+  X(14, 2, 7, 0x2AAAu, 4, 0x6420u)    /* 4 | 4 4 | 4 4 | 4 4      */
 
 __global__ void __launch_bounds__(BT) batch_kernel(const pacg_batch_args_t a) {
   __shared__ float2 lds[BLD];
@@ -374,21 +417,21 @@ __global__ void __launch_bounds__(BT) batch_kernel(const pacg_batch_args_t a) {
     else if (t < 256) twl[t - 128] = tw[t - 128];
   }
   __syncthreads();
-  for (uint32_t i = blockIdx.x; i < a.n_items; i += gridDim.x) {
+  uint32_t i = a.wg_first[blockIdx.x];
+  const uint32_t end = a.wg_first[blockIdx.x + 1];
+  while (i < end) {
     const pacg_batch_item_t it = a.items[i];
     const int key = (int)it.log2n * 4 + (int)it.variant;
     switch (key) {
-#define PACG_CASE(M, V, NS, SC, NPH, PC)                                    \
-  case M * 4 + V:                                                          \
-    static_assert(Shape<NS, SC, NPH, PC>::LGP == M, "variant length");    \
-    if constexpr (Shape<NS, SC, NPH, PC>::P >= BT * 8)                     \
-      batch_item_vt<Shape<NS, SC, NPH, PC>>(a, it, lds, twh, twl);    \
-    else                                                                   \
-      batch_item<Shape<NS, SC, NPH, PC>>(a, it, lds, twh, twl);       \
+#define PACG_CASE(M, V, NS, SC, NPH, PC)                                 \
+  case M * 4 + V:                                                       \
+    static_assert(Shape<NS, SC, NPH, PC>::LGP == M, "variant length"); \
+    i = batch_run<Shape<NS, SC, NPH, PC>>(a, i, end, lds, twh, twl);   \
     break;
       PACG_BATCH_VARIANTS(PACG_CASE)
 #undef PACG_CASE
       default:
+        i = end;  // no such variant (the host never builds one): leave the list
         break;
     }
   }
@@ -414,7 +457,7 @@ const pacg_batch_variant_t pacg_batch_variants[] = {PACG_BATCH_VARIANTS(PACG_ROW
 const int pacg_num_batch_variants = sizeof(pacg_batch_variants) / sizeof(pacg_batch_variants[0]);
 
 int pacg_batch_launch(const pacg_batch_args_t* a, int blocks, void* stream) {
-  if (a->n_items == 0) return 0;
+  if (a->n_items == 0 || blocks < 1) return 0;
   hipLaunchKernelGGL(pacg::batch_kernel, dim3(blocks), dim3(pacg::BT), 0, (hipStream_t)stream, *a);
   return (int)hipGetLastError();
 }

@@ -134,7 +134,8 @@ int pacg_profile_convolve(pacg_plan* plan, const void* x_re, const void* x_im, v
                           float* launch_ms, const char** names, int max_launches);
 
 /* ---- batched convolutions of uneven lengths (one persistent launch) --------
- * count pairs (x_i, k_i) of lengths 2^m_i, 8 <= m_i <= 14, FP32 split complex,
+ * count pairs (x_i, k_i) of lengths 2^m_i, 8 <= m_i <= 14, fewer than 2^30 points in
+ * all, FP32 split complex,
  * packed back to back in input order (pair i at pacg_batch_offset(b, i) in each
  * of the six arrays).  y_i = (1/n_i) conj(A_i)[(A_i^T k_i) .* (A_i^T x_i)] with a
  * per-length schedule chosen from compiled variants (pacg_batch_autotune). */
