@@ -73,12 +73,13 @@ static int per_item(int m, int v) {
   return g < 1 ? 1 : g;
 }
 
-/* Relative cost of an item of length 2^m with TPL threads per pair: every
- * thread of the workgroup runs the whole item (a ragged item computes its
- * missing pairs too and stores nothing), so the cost is the item's points
- * (1024 x points per thread) times a per-point cost that grows with the
- * number of stages. */
-static double item_cost(int m, int tpl) { return (double)BATCH_THREADS * (double)((1 << m) / tpl) * (4.0 + m); }
+/* Cost of an item: every thread of the workgroup runs the whole item (a
+ * ragged item computes its missing pairs too and stores nothing), so it is the
+ * variant's measured time per item, whatever the item's pair count. */
+static double item_cost(int m, int v) {
+  const pacg_batch_variant_t* r = variant_row(m, v);
+  return r ? (double)r->cost : 1.0;
+}
 
 typedef struct {
   double cost;
@@ -108,8 +109,7 @@ static int deal_items(const pacg_batch_item_t* items, int n, int w, pacg_batch_i
     return pacg_fail(PACG_ERR_NOMEM, "out of host memory");
   }
   for (int i = 0; i < n; ++i) {
-    const pacg_batch_variant_t* r = variant_row(items[i].log2n, items[i].variant);
-    ci[i].cost = item_cost(items[i].log2n, r ? r->tpl : 1);
+    ci[i].cost = item_cost(items[i].log2n, items[i].variant);
     ci[i].idx = i;
   }
   qsort(ci, (size_t)n, sizeof(cost_idx_t), by_cost_desc);
@@ -335,42 +335,63 @@ int pacg_batch_convolve(pacg_batch* b, const float* xr, const float* xi, const f
   return launch(b, xr, xi, kr, ki, yr, yi);
 }
 
+/* whole-batch launch time in ms: one warm launch, then the mean of `reps` */
+static int time_batch(pacg_batch* b, const float* xr, const float* xi, const float* kr, const float* ki, float* yr,
+                      float* yi, int reps, void* e0, void* e1, float* ms) {
+  int rc = launch(b, xr, xi, kr, ki, yr, yi);
+  if (rc) return rc;
+  pacg_event_record(e0, b->stream);
+  for (int r = 0; r < reps && !rc; ++r) rc = launch(b, xr, xi, kr, ki, yr, yi);
+  pacg_event_record(e1, b->stream);
+  if (pacg_stream_sync(b->stream)) return pacg_fail(PACG_ERR_HIP, "autotune timing failed");
+  if (rc) return rc;
+  *ms = 0.0f;
+  pacg_event_elapsed_ms(ms, e0, e1);
+  *ms /= (float)reps;
+  return PACG_OK;
+}
+
+/* Coordinate descent over the lengths present, longest first: each variant of
+ * a length is timed on the whole batch (the other lengths at their current
+ * variants), because what a variant costs depends on how its items fill the
+ * workgroups beside the others.  Two sweeps, each variant keeping its faster
+ * measurement; a length keeps its current variant unless another is faster by
+ * more than 1 %. */
 int pacg_batch_autotune(pacg_batch* b, const float* xr, const float* xi, const float* kr, const float* ki, float* yr,
                         float* yi, int reps) {
   if (!b || !xr || !xi || !kr || !ki || !yr || !yi || reps < 1) return -pacg_fail(PACG_ERR_ARG, "bad argument");
   void *e0 = NULL, *e1 = NULL;
-  if (pacg_event_create(&e0) || pacg_event_create(&e1)) return -pacg_fail(PACG_ERR_HIP, "event creation failed");
-  int changed = 0, rc = PACG_OK;
-  for (int m = LMIN; m <= LMAX && !rc; ++m) {
-    int present = 0;
-    for (int i = 0; i < b->count; ++i) present |= b->log2n[i] == m;
-    if (!present) continue;
-    /* two rounds over the variants, each keeps its faster round (as the plan
-     * autotuner); variant 0 unless another is faster by more than 2 % */
-    enum { MAXV = 8 };
-    float vms[MAXV];
-    const int nv = pacg_batch_num_variants(m) < MAXV ? pacg_batch_num_variants(m) : MAXV;
-    for (int v = 0; v < nv; ++v) vms[v] = 1e30f;
-    for (int round = 0; round < 2 && !rc; ++round) {
-      for (int v = 0; v < nv && !rc; ++v) {
+  if (pacg_event_create(&e0) || pacg_event_create(&e1)) {
+    if (e0) pacg_event_destroy(e0);
+    return -pacg_fail(PACG_ERR_HIP, "event creation failed");
+  }
+  int start[LMAX + 1];
+  for (int m = 0; m <= LMAX; ++m) start[m] = b->variant[m];
+  int rc = PACG_OK;
+  for (int sweep = 0; sweep < 2 && !rc; ++sweep) {
+    for (int m = LMAX; m >= LMIN && !rc; --m) {
+      int present = 0;
+      for (int i = 0; i < b->count && !present; ++i) present = b->log2n[i] == m;
+      const int nv = pacg_batch_num_variants(m);
+      if (!present || nv < 2) continue;
+      enum { MAXV = 8 };
+      float vms[MAXV];
+      const int cur = b->variant[m];
+      for (int v = 0; v < nv && v < MAXV && !rc; ++v) {
         b->variant[m] = v;
-        if ((rc = build_items(b, 1u << m))) break;
-        if ((rc = launch(b, xr, xi, kr, ki, yr, yi))) break;
-        pacg_event_record(e0, b->stream);
-        for (int r = 0; r < reps && !rc; ++r) rc = launch(b, xr, xi, kr, ki, yr, yi);
-        pacg_event_record(e1, b->stream);
-        if (pacg_stream_sync(b->stream)) rc = pacg_fail(PACG_ERR_HIP, "autotune timing failed");
-        float ms = 0.0f;
-        pacg_event_elapsed_ms(&ms, e0, e1);
-        if (!rc && ms < vms[v]) vms[v] = ms;
+        if ((rc = build_items(b, ~0u))) break;
+        float t1 = 0.0f, t2 = 0.0f;
+        if ((rc = time_batch(b, xr, xi, kr, ki, yr, yi, reps, e0, e1, &t1))) break;
+        if ((rc = time_batch(b, xr, xi, kr, ki, yr, yi, reps, e0, e1, &t2))) break;
+        vms[v] = t1 < t2 ? t1 : t2;
       }
+      if (rc) break;
+      int best = cur;
+      for (int v = 0; v < nv && v < MAXV; ++v)
+        if (vms[v] < vms[best]) best = v;
+      if (vms[cur] <= 1.01f * vms[best]) best = cur;
+      b->variant[m] = best;
     }
-    int best = 0;
-    for (int v = 1; v < nv; ++v)
-      if (vms[v] < vms[best]) best = v;
-    if (vms[0] <= 1.02f * vms[best]) best = 0;
-    if (best != 0) ++changed;
-    b->variant[m] = best;
   }
   pacg_event_destroy(e0);
   pacg_event_destroy(e1);
@@ -381,6 +402,8 @@ int pacg_batch_autotune(pacg_batch* b, const float* xr, const float* xi, const f
   }
   if (rc) return -rc;
   if (rc2) return -rc2;
+  int changed = 0;
+  for (int m = LMIN; m <= LMAX; ++m) changed += b->variant[m] != start[m];
   return changed;
 }
 

@@ -37,6 +37,7 @@ typedef struct {
   int log2n, variant, nstages;
   int radix[7];                      /* stage 0 first */
   int tpl;                           /* threads per convolution */
+  int cost;                          /* time of one item on one workgroup, 0.1 us (MI355X) */
 } pacg_batch_variant_t;
 
 extern const pacg_batch_variant_t pacg_batch_variants[];

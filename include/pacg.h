@@ -148,8 +148,10 @@ void* pacg_batch_stream(const pacg_batch* batch);
 int pacg_batch_items(const pacg_batch* batch);
 int pacg_batch_convolve(pacg_batch* batch, const float* x_re, const float* x_im, const float* k_re, const float* k_im,
                         float* y_re, float* y_im);
-/* Times every compiled schedule of each length present on these buffers and
- * keeps the fastest; returns how many lengths left variant 0, or -error. */
+/* Chooses a compiled schedule per length present: each is timed on the whole
+ * batch (these buffers), the other lengths at their current schedules, over
+ * two sweeps, and the fastest is kept.  Returns how many lengths changed
+ * schedule, or -error.  Drops a captured graph. */
 int pacg_batch_autotune(pacg_batch* batch, const float* x_re, const float* x_im, const float* k_re,
                         const float* k_im, float* y_re, float* y_im, int reps);
 int pacg_batch_set_variant(pacg_batch* batch, int log2n, int variant);
