@@ -65,6 +65,7 @@ def _load():
     L.pacg_convolve.argtypes = [_vp, _vp, _vp, _vp, _vp]
     L.pacg_convolve_pair.argtypes = [_vp, _vp, _vp, _vp, _vp]
     L.pacg_convolve_fft_based.argtypes = [_vp, _vp, _vp, _vp, _vp]
+    L.pacg_convolve_fft_based_noalloc.argtypes = [_vp, _vp, _vp, _vp, _vp]
     L.pacg_graph_capture_convolve.argtypes = [_vp, _vp, _vp, _vp, _vp]
     L.pacg_graph_replay.argtypes = [_vp]
     L.pacg_sync.argtypes = [_vp]

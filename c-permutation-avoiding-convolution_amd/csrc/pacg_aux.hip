@@ -10,6 +10,9 @@
 
 #include <stdint.h>
 
+#include <mutex>
+
+#include "pacg.h"
 #include "pacg_internal.h"
 
 namespace {
@@ -263,3 +266,32 @@ int pacg_graph_launch(void* exec, void* stream) { return (int)hipGraphLaunch((hi
 int pacg_graph_destroy(void* exec) { return (int)hipGraphExecDestroy((hipGraphExec_t)exec); }
 
 }  // extern "C"
+
+// Scratch of the allocation-free entry point (pacg_convolve_fft_based_noalloc):
+// a fixed array in the code object, one copy on every GPU the library runs
+// on, held until the process ends.  Its name is a host-side stand-in for a
+// device address: host code uses only the address hipGetSymbolAddress gives
+// for the current device, resolved once per device and kept here.
+static __device__ __align__(256) unsigned char scratch[PACG_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices];
+}  // namespace
+
+extern "C" int pacg_noalloc_scratch(void** p) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return (int)e;
+  if (dev < 0 || dev >= kMaxDevices) return (int)hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* q = nullptr;
+    e = hipGetSymbolAddress(&q, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return (int)e;
+    g_scratch_ptr[dev] = q;
+  }
+  *p = g_scratch_ptr[dev];
+  return 0;
+}

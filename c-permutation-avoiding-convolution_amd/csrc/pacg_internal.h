@@ -165,6 +165,11 @@ int pacg_pointwise(void* zre, void* zim, const void* wre, const void* wim, int p
 int pacg_convert(const void* xre, const void* xim, int xprec, void* yre, void* yim, int yprec,
                  uint64_t n, double scale, void* stream);
 
+/* Device address, on the current device, of the fixed scratch array of the
+ * allocation-free entry point (PACG_NOALLOC_SCRATCH_BYTES, include/pacg.h);
+ * resolved with hipGetSymbolAddress once per device.  Returns a HIP error code. */
+int pacg_noalloc_scratch(void** p);
+
 /* HBM probe: mode 0 read `bytes` of src, 1 write `bytes` to dst, 2 copy src -> dst
  * (16-byte accesses; the read mode writes dst[0] only in an impossible case). */
 int pacg_bw_probe(int mode, void* dst, const void* src, uint64_t bytes, void* stream);

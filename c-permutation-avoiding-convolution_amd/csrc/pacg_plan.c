@@ -521,6 +521,13 @@ int pacg_plan_create(pacg_plan** out, const pacg_desc* d) {
     if (d->precision == PACG_PREC_F64) sp = PACG_F64;
     pl->d.spectrum_precision = sp;
   }
+  /* set-up of the allocation-free entry point on this device: resolve the
+   * address of its fixed scratch now (a failure here is reported by that
+   * entry point, which resolves it again) */
+  {
+    void* scr = NULL;
+    (void)pacg_noalloc_scratch(&scr);
+  }
   *out = pl;
   pacg_clear_error();
   return PACG_OK;
@@ -981,8 +988,10 @@ static void perm_desc(const pacg_plan* pl, pacg_perm_desc_t* pd) {
   }
 }
 
-int pacg_execute(pacg_plan* pl, int op, const void* ir, const void* ii, void* orr, void* oi) {
-  if (!pl || !ir || !ii || !orr || !oi) return pacg_fail(PACG_ERR_ARG, "pacg_execute: NULL argument");
+/* pacg_execute with the digit-reversal scratch (sr, si) of n elements each,
+ * used by the ordered transforms (FFT, IFFT, FFT2) only */
+static int execute_with(pacg_plan* pl, int op, const void* ir, const void* ii, void* orr, void* oi, void* sr,
+                        void* si) {
   int prec = pidx(pl);
   double inv_n = (double)pl->batch / (double)pl->n;
   pacg_perm_desc_t pd;
@@ -998,20 +1007,25 @@ int pacg_execute(pacg_plan* pl, int op, const void* ir, const void* ii, void* or
       return run_all(pl, prec, PACG_K_DITC, ir, ii, orr, oi, inv_n);
     case PACG_OP_FFT:
     case PACG_OP_IFFT:
-      if ((rc = ensure_scratch(pl))) return rc;
       perm_desc(pl, &pd);
-      HIPCHK(pacg_permute(ir, ii, pl->scr_re, pl->scr_im, prec, &pd, 0, pl->stream), "permute");
-      return run_all(pl, prec, op == PACG_OP_FFT ? PACG_K_DIT : PACG_K_DITC, pl->scr_re, pl->scr_im, orr, oi,
+      HIPCHK(pacg_permute(ir, ii, sr, si, prec, &pd, 0, pl->stream), "permute");
+      return run_all(pl, prec, op == PACG_OP_FFT ? PACG_K_DIT : PACG_K_DITC, sr, si, orr, oi,
                      op == PACG_OP_FFT ? 1.0 : inv_n);
     case PACG_OP_FFT2:
-      if ((rc = ensure_scratch(pl))) return rc;
       perm_desc(pl, &pd);
-      if ((rc = run_all(pl, prec, PACG_K_DIF, ir, ii, pl->scr_re, pl->scr_im, 1.0))) return rc;
-      HIPCHK(pacg_permute(pl->scr_re, pl->scr_im, orr, oi, prec, &pd, 1, pl->stream), "permute");
+      if ((rc = run_all(pl, prec, PACG_K_DIF, ir, ii, sr, si, 1.0))) return rc;
+      HIPCHK(pacg_permute(sr, si, orr, oi, prec, &pd, 1, pl->stream), "permute");
       return PACG_OK;
     default:
       return pacg_fail(PACG_ERR_ARG, "pacg_execute: unknown op %d", op);
   }
+}
+
+int pacg_execute(pacg_plan* pl, int op, const void* ir, const void* ii, void* orr, void* oi) {
+  if (!pl || !ir || !ii || !orr || !oi) return pacg_fail(PACG_ERR_ARG, "pacg_execute: NULL argument");
+  int rc;
+  if ((op == PACG_OP_FFT || op == PACG_OP_IFFT || op == PACG_OP_FFT2) && (rc = ensure_scratch(pl))) return rc;
+  return execute_with(pl, op, ir, ii, orr, oi, pl->scr_re, pl->scr_im);
 }
 
 int pacg_set_kernel(pacg_plan* pl, const double* kr, const double* ki, int on_device) {
@@ -1109,13 +1123,33 @@ int pacg_convolve_pair(pacg_plan* pl, void* xr, void* xi, void* hr, void* hi) {
   return run_conv(pl, prec, xr, xi, xr, xi, hr, hi, prec);
 }
 
+/* FFT-based convolution (convolution.c:12-29) with the scratch (sr, si) */
+static int fft_based_with(pacg_plan* pl, void* xr, void* xi, void* hr, void* hi, void* sr, void* si) {
+  int prec = pidx(pl), rc;
+  if ((rc = execute_with(pl, PACG_OP_FFT, xr, xi, xr, xi, sr, si))) return rc;
+  if ((rc = execute_with(pl, PACG_OP_FFT, hr, hi, hr, hi, sr, si))) return rc;
+  HIPCHK(pacg_pointwise(xr, xi, hr, hi, prec, prec, (uint64_t)pl->n, 1.0, pl->stream), "pointwise");
+  return execute_with(pl, PACG_OP_IFFT, xr, xi, xr, xi, sr, si);
+}
+
 int pacg_convolve_fft_based(pacg_plan* pl, void* xr, void* xi, void* hr, void* hi) {
   if (!pl || !xr || !xi || !hr || !hi) return pacg_fail(PACG_ERR_ARG, "pacg_convolve_fft_based: NULL argument");
-  int prec = pidx(pl), rc;
-  if ((rc = pacg_execute(pl, PACG_OP_FFT, xr, xi, xr, xi))) return rc;
-  if ((rc = pacg_execute(pl, PACG_OP_FFT, hr, hi, hr, hi))) return rc;
-  HIPCHK(pacg_pointwise(xr, xi, hr, hi, prec, prec, (uint64_t)pl->n, 1.0, pl->stream), "pointwise");
-  return pacg_execute(pl, PACG_OP_IFFT, xr, xi, xr, xi);
+  int rc;
+  if ((rc = ensure_scratch(pl))) return rc;
+  return fft_based_with(pl, xr, xi, hr, hi, pl->scr_re, pl->scr_im);
+}
+
+/* The same with the library's fixed device scratch: no allocation. */
+int pacg_convolve_fft_based_noalloc(pacg_plan* pl, void* xr, void* xi, void* hr, void* hi) {
+  if (!pl || !xr || !xi || !hr || !hi)
+    return pacg_fail(PACG_ERR_ARG, "pacg_convolve_fft_based_noalloc: NULL argument");
+  const size_t half = (size_t)pl->n * esize(pidx(pl));
+  if (2 * half > (size_t)PACG_NOALLOC_SCRATCH_BYTES)
+    return pacg_fail(PACG_ERR_SIZE, "pacg_convolve_fft_based_noalloc: %lld points need %zu bytes of scratch, more "
+                     "than the fixed %u", (long long)pl->n, 2 * half, (unsigned)PACG_NOALLOC_SCRATCH_BYTES);
+  void* base = NULL;
+  HIPCHK(pacg_noalloc_scratch(&base), "scratch address");
+  return fft_based_with(pl, xr, xi, hr, hi, base, (char*)base + half);
 }
 
 int pacg_graph_capture_convolve(pacg_plan* pl, const void* xr, const void* xi, void* yr, void* yi) {

@@ -108,6 +108,20 @@ int pacg_convolve_pair(pacg_plan* plan, void* x_re, void* x_im, void* h_re, void
 /* FFT-based (permuting) convolution: h <- F h, x <- ifft(F x .* F h) (convolution.c:12-29). */
 int pacg_convolve_fft_based(pacg_plan* plan, void* x_re, void* x_im, void* h_re, void* h_im);
 
+/* Allocation-free variant of pacg_convolve_fft_based, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.  Same
+ * kernels and results as pacg_convolve_fft_based; the digit-reversal scratch
+ * is a fixed device array built into the library instead of plan memory
+ * allocated on first use: PACG_NOALLOC_SCRATCH_BYTES on every GPU the library
+ * runs on, held until the process ends.  A plan whose scratch (2 x elements x
+ * element size: 16 bytes per FP64 point, 8 per FP32) exceeds it is rejected
+ * with PACG_ERR_SIZE.  All calls on one GPU share that array, and it keeps the
+ * last call's contents: calls that overlap on one GPU (several streams or
+ * threads at once) race.  The call itself makes no hipMalloc / hipFree. */
+#define PACG_NOALLOC_SCRATCH_BYTES (32u << 20)
+int pacg_convolve_fft_based_noalloc(pacg_plan* plan, void* x_re, void* x_im, void* h_re, void* h_im);
+
 /* Capture pacg_convolve(x -> y) on the plan stream into a HIP graph and replay it. */
 int pacg_graph_capture_convolve(pacg_plan* plan, const void* x_re, const void* x_im, void* y_re, void* y_im);
 int pacg_graph_replay(pacg_plan* plan);
