@@ -114,22 +114,62 @@ def conv_alg_bytes(launches, n, e, s):
 
 def measured_traffic(variant):
     """HBM bytes per launch of `variant` from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs,
-    each counter scaled by what it reads for a copy kernel of known size (profiles/traffic_3d256_fp32.json)."""
-    path = os.path.join(ROOT, "profiles", "traffic_3d256_fp32.json")
+    each counter scaled by what it reads for a copy kernel of known size (profiles/traffic_3d256_fp32.json),
+    with where the figure comes from: (bytes or None, source).  The counters are not read in this run."""
+    rel = os.path.join("profiles", "traffic_3d256_fp32.json")
+    path = os.path.join(ROOT, rel)
     if not os.path.exists(path):
-        return None
+        return None, "no %s" % rel
     with open(path) as f:
-        entry = json.load(f).get("kernels", {}).get(variant)
-    return float(entry["traffic_bytes"]) if entry else None
+        d = json.load(f)
+    entry = d.get("kernels", {}).get(variant)
+    if not entry:
+        return None, "%s has no entry for %s (the dominant kernel of this run); regenerate it" % (rel, variant)
+    return float(entry["traffic_bytes"]), "%s (%s; not measured in this run)" % (rel, "; ".join(d.get("commands", [])))
 
 
-def cpu_baseline(x, k, reps):
-    """The reference's permutation_avoiding_convolution_3d (r = 4: 256 = 4^4) on one host core."""
+def host_cpu_info():
+    """CPU model (/proc/cpuinfo), cores of the host and of this process, and the compile flags of the
+    CPU reference library (oracle/Makefile REF_CFLAGS)."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    flags = None
+    try:
+        with open(os.path.join(ROOT, "oracle", "Makefile")) as f:
+            for line in f:
+                if line.startswith("REF_CFLAGS"):
+                    flags = line.split("=", 1)[1].strip()
+    except OSError:
+        pass
+    return {"cpu_model": model, "host_cores": os.cpu_count(), "process_cores": len(os.sched_getaffinity(0)),
+            "compile_flags": flags}
+
+
+def _load_synth():
+    """synth.py by path (numpy only): the CPU child process loads no GPU library."""
+    spec = importlib.util.spec_from_file_location("pacg_synth_cpu", os.path.join(PKG_DIR, "synth.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def _cpu_baseline_child(reps, core, q):
+    """Pinned to one core (the reference's protocol: taskset -c 0, bench1d_forward_fft.sh:109), the
+    reference's permutation_avoiding_convolution_3d (r = 4: 256 = 4^4) on the headline's inputs:
+    one warm-up and `reps` timed calls, h re-copied before each, untimed (bench1d_convolution.c:238-260)."""
+    os.sched_setaffinity(0, {core})
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     from oracle import Oracle, RefLib  # checker / baseline only
 
+    x, k = _load_synth().workload_inputs("3d_256cubed")
     kind = "reference" if RefLib.available() else "port"
-    n = x.size
     re = np.ascontiguousarray(x.real)
     im = np.ascontiguousarray(x.imag)
     hre0 = np.ascontiguousarray(k.real)
@@ -159,11 +199,28 @@ def cpu_baseline(x, k, reps):
             dt = time.perf_counter() - t0
             if it > 0:
                 times.append(dt)
+    q.put((kind, times, sorted(os.sched_getaffinity(0))))
+
+
+def cpu_baseline(reps):
+    """The reference's 256^3 convolution on one pinned host core, in a child process (see
+    _cpu_baseline_child), with the host's CPU model, core count and the library's compile flags."""
+    import multiprocessing as mp
+
+    core = min(os.sched_getaffinity(0))
+    ctx = mp.get_context("spawn")
+    q = ctx.Queue()
+    p_ = ctx.Process(target=_cpu_baseline_child, args=(reps, core, q))
+    p_.start()
+    kind, times, pinned = q.get(timeout=1800)
+    p_.join(timeout=60)
     mean = float(np.mean(times))
-    return {"value": 1.0 / mean, "unit": "conv/s", "cores": 1, "kind": kind,
-            "sample": "256^3 complex FP64 PA convolution, radix 4, 1 warm-up + %d timed runs "
-                      "(mean %.3f s +/- %.3f s), h re-copied before each call" % (reps, mean, float(np.std(times))),
-            "n": int(n)}
+    return {"value": 1.0 / mean, "unit": "conv/s", "cores": 1, "kind": kind, "pinned_cores": pinned,
+            "warmup_calls": 1, "timed_calls": len(times), "mean_s": mean, "std_s": float(np.std(times)),
+            "sample": "256^3 complex FP64 PA convolution (h transformed in the call: 3 transforms), radix 4, "
+                      "one core pinned with sched_setaffinity, 1 warm-up + %d timed calls (mean %.3f s +/- %.3f s), "
+                      "h re-copied before each call, untimed" % (len(times), mean, float(np.std(times))),
+            "n": N1 ** 3, **host_cpu_info()}
 
 
 def cpu_baseline_workloads(synth, gpu):
@@ -318,12 +375,16 @@ def bench_fp64_workloads(pacg, synth, steps, warmup):
         pl.autotune(dx, dy, 3)
         ms = time_convs(pacg, pl, dx, dy, steps, warmup)
         ms_graph = time_graph(pacg, pl, dx, dy, steps, warmup)
+        prof = profile_launches(pacg, pl, dx, dy, max(5, steps // 2))
         b_alg = (2 * d - 1) * 2 * n * 16 + n * 16
         y = dy.numpy()
         ref = np.fft.ifftn(np.fft.fftn(x) * np.fft.fftn(k))
         entry = {"ms_per_conv": ms, "conv_per_s": 1000.0 / ms, "launches": pl.launches_per_convolve(),
                  "alg_bytes": b_alg, "frac": b_alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                 "rel_l2_vs_numpy": float(np.linalg.norm(y - ref) / np.linalg.norm(ref)), "plan": pl.describe(),
+                 "gflops": 2 * 5 * n * np.log2(n) / (ms * 1e-3) / 1e9,
+                 "rel_l2_vs_numpy": float(np.linalg.norm(y - ref) / np.linalg.norm(ref)),
+                 "max_abs_vs_numpy": float(np.abs(y - ref).max()), "plan": pl.describe(),
+                 "kernels": [{"kernel": nm, "ms": t} for nm, t in prof],
                  "graph_replay_ms_per_conv": ms_graph}
         del pl, dx, dy
         # drop-in entry point, host arrays in and out (the reference's radix: 2 for both)
@@ -371,6 +432,7 @@ def bench_mfma(pacg, synth, steps, warmup):
         b_alg = 3 * 2 * n * 8 + n * 8
         out[name] = {"ms_per_conv": ms, "conv_per_s": 1000.0 / ms, "launches": pl.launches_per_convolve(),
                      "frac_of_B_alg": b_alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                     "gflops": 2 * 5 * n * np.log2(n) / (ms * 1e-3) / 1e9,
                      "rel_l2": float(np.linalg.norm(y - ref) / np.linalg.norm(ref)),
                      "max_abs": float(np.abs(y - ref).max()), "plan": pl.describe(),
                      "kernels": [{"kernel": nm, "ms": t} for nm, t in prof]}
@@ -393,6 +455,169 @@ def bench_hbm_probe(pacg):
     return out
 
 
+# The reference's published per-call seconds (one host core, FP64), from its plot scripts:
+# butterflies (ufft in the forward_fft programs / uifft in the backward_fft programs),
+# per-element-butterfly-cost-plot.py:17-34, and the digit-reversal permutation timed in the same two
+# programs, per-element-permutation-cost-plot.py:17-34.  Keyed by (dims, radix) -> {n: (fwd, bwd)}.
+REF_PUBLISHED = {
+    "butterflies": {
+        ("1d", 2): {2**20: (.0170, .0140), 2**22: (.0780, .0650), 2**24: (.3320, .5065), 2**26: (1.5445, 2.5100),
+                    2**28: (6.205, 11.1795)},
+        ("1d", 4): {2**20: (.0135, .011), 2**22: (.0535, .0550), 2**24: (.249, .330), 2**26: (1.0815, 1.563),
+                    2**28: (4.5890, 6.9505)},
+        ("2d", 2): {2**20: (.029, .0355), 2**22: (.2000, .1420), 2**24: (1.0385, 1.1605), 2**26: (4.591, 5.829),
+                    2**28: (20.0615, 23.7825)},
+        ("2d", 4): {2**20: (.0280, .0340), 2**24: (1.0295, 1.1345), 2**28: (19.8865, 23.5950)},
+        ("3d", 2): {2**21: (.0054, .0795), 2**24: (0.7220, 0.8175), 2**27: (6.528, 10.0875)},
+        ("3d", 4): {2**24: (0.7035, 0.7785)},
+    },
+    "permutation": {
+        ("1d", 2): {2**20: (.011, .016), 2**22: (.0740, .0930), 2**24: (.3745, .8435), 2**26: (1.687, 4.1915),
+                    2**28: (5.5190, 17.3465)},
+        ("1d", 4): {2**20: (.011, .0135), 2**22: (.068, .090), 2**24: (.3485, .8315), 2**26: (1.6500, 4.1445),
+                    2**28: (5.5645, 17.3075)},
+        ("2d", 2): {2**20: (.0015, .002), 2**22: (.006, .007), 2**24: (.0650, .0700), 2**26: (.2960, .3405),
+                    2**28: (1.250, 1.4735)},
+        ("2d", 4): {2**20: (.001, .0015), 2**24: (.0625, .0650), 2**28: (1.2300, 1.3915)},
+        ("3d", 2): {2**21: (.0020, .0020), 2**24: (.0215, .0280), 2**27: (.184, .270)},
+        ("3d", 4): {2**24: (.0255, .0295)},
+    },
+}
+
+
+def _timed_calls(pacg, stream, reset, call, reps):
+    """Mean ms of `reps` calls, each timed alone with HIP events on `stream` after an untimed `reset`
+    (the reference re-copies its inputs outside the timed region, bench1d_convolution.c:238-260)."""
+    t = pacg.Timer(stream)
+    reset()
+    call()  # warm-up
+    ms = []
+    for _ in range(reps):
+        reset()
+        t.start()
+        call()
+        ms.append(t.stop())
+    return float(np.mean(ms)), float(np.std(ms))
+
+
+def _errs(y, ref):
+    return {"rel_l2": float(np.linalg.norm(y - ref) / np.linalg.norm(ref)), "max_abs": float(np.abs(y - ref).max())}
+
+
+def bench_reference_semantics(pacg, x, k, steps):
+    """Device-resident convolutions with the reference's own semantics at 256^3: every call transforms
+    h as well (h <- A^T h, convolution.c:45-53), so three transforms per call, as in the CPU baseline
+    (pacg_convolve_pair), in FP32 and FP64; x and h reset from device copies before each call,
+    untimed.  Also the FFT-based convolution (convolution.c:12-29: three ordered transforms, three
+    digit-reversal permutations and the product) on the same inputs in FP64."""
+    n = x.size
+    ref = np.fft.ifftn(np.fft.fftn(x) * np.fft.fftn(k))
+    out = {}
+    for prec, e, name in ((pacg.PREC_F32, 8, "fp32"), (pacg.PREC_F64, 16, "fp64")):
+        dt = np.float32 if prec == pacg.PREC_F32 else np.float64
+        pl = pacg.Plan([N1] * 3, schedules=SCHEDS32, precision=prec)
+        x0, h0 = pacg.DeviceComplex.from_numpy(x, dt), pacg.DeviceComplex.from_numpy(k, dt)
+        dx, dh = pacg.DeviceComplex.from_numpy(x, dt), pacg.DeviceComplex.from_numpy(k, dt)
+        nb = n * np.dtype(dt).itemsize
+
+        def reset():
+            for dst, src in ((dx, x0), (dh, h0)):
+                pacg.lib.pacg_memcpy_d2d(dst.re.ptr, src.re.ptr, nb, pl.stream)
+                pacg.lib.pacg_memcpy_d2d(dst.im.ptr, src.im.ptr, nb, pl.stream)
+
+        ms, sd = _timed_calls(pacg, pl.stream, reset, lambda: pl.convolve_pair(dx, dh), max(5, steps // 2))
+        y = dx.numpy()
+        b = 5 * 2 * n * e + n * e + 3 * 2 * n * e  # x: 5 round trips + spectrum read; h: 3 round trips
+        out["pa_" + name] = {"ms_per_conv": ms, "std_ms": sd, "conv_per_s": 1000.0 / ms, "alg_bytes": b,
+                             "frac": b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                             "gflops": 3 * 5 * n * np.log2(n) / (ms * 1e-3) / 1e9, **_errs(y, ref)}
+        if prec == pacg.PREC_F64:
+            ms_f, sd_f = _timed_calls(pacg, pl.stream, reset, lambda: pl.convolve_fft_based(dx, dh),
+                                      max(5, steps // 2))
+            y = dx.numpy()
+            out["fft_based_fp64"] = {"ms_per_conv": ms_f, "std_ms": sd_f, "conv_per_s": 1000.0 / ms_f,
+                                     "gflops": 3 * 5 * n * np.log2(n) / (ms_f * 1e-3) / 1e9, **_errs(y, ref),
+                                     "fft_based_over_pa": ms_f / ms}
+        del pl, x0, h0, dx, dh
+    return out
+
+
+def bench_fft_based_1d(pacg, steps, log2n=24):
+    """FFT-based against permutation-avoiding convolution (both with the reference's semantics:
+    h transformed in the call) for one 1D FP64 array of 2^24 points, default radix-4 schedule,
+    U(-1,1) inputs: what skipping the three permutations is worth on the device."""
+    n = 1 << log2n
+    g = np.random.default_rng(5)
+    x = g.uniform(-1, 1, n) + 1j * g.uniform(-1, 1, n)
+    k = g.uniform(-1, 1, n) + 1j * g.uniform(-1, 1, n)
+    ref = np.fft.ifft(np.fft.fft(x) * np.fft.fft(k))
+    pl = pacg.Plan([n], precision=pacg.PREC_F64)
+    x0, h0 = pacg.DeviceComplex.from_numpy(x), pacg.DeviceComplex.from_numpy(k)
+    dx, dh = pacg.DeviceComplex.from_numpy(x), pacg.DeviceComplex.from_numpy(k)
+    nb = n * 8
+
+    def reset():
+        for dst, src in ((dx, x0), (dh, h0)):
+            pacg.lib.pacg_memcpy_d2d(dst.re.ptr, src.re.ptr, nb, pl.stream)
+            pacg.lib.pacg_memcpy_d2d(dst.im.ptr, src.im.ptr, nb, pl.stream)
+
+    out = {"n": n, "plan": pl.describe()}
+    for name, fn in (("pa", pl.convolve_pair), ("fft_based", pl.convolve_fft_based)):
+        ms, sd = _timed_calls(pacg, pl.stream, reset, lambda: fn(dx, dh), max(5, steps // 2))
+        out[name] = {"ms_per_conv": ms, "std_ms": sd, "gflops": 3 * 5 * n * log2n / (ms * 1e-3) / 1e9,
+                     **_errs(dx.numpy(), ref)}
+    out["fft_based_over_pa"] = out["fft_based"]["ms_per_conv"] / out["pa"]["ms_per_conv"]
+    del pl, x0, h0, dx, dh
+    return out
+
+
+def bench_per_element(pacg, reps=3):
+    """Device counterparts of the reference's only published numbers: FP64 per-call and per-element
+    times of ufft (A), uifft (conj(A) / n) and the digit-reversal permutation (gather into another
+    array) at the sizes of its plot scripts, pure radix 2 and 4 as there, device-resident, HIP
+    events; the reference's seconds beside them (REF_PUBLISHED, one host core)."""
+    cases = []
+    for r in (2, 4):
+        for t in (20, 22, 24, 26, 28):
+            cases.append(("1d", r, [1 << t]))
+        for t in ((10, 12, 14) if r == 4 else (10, 11, 12, 13, 14)):
+            cases.append(("2d", r, [1 << t] * 2))
+    cases += [("3d", 2, [1 << 7] * 3), ("3d", 2, [1 << 8] * 3), ("3d", 2, [1 << 9] * 3), ("3d", 4, [1 << 8] * 3)]
+    zeros = None
+    out = []
+    for dims_name, r, dims in cases:
+        n = int(np.prod(dims))
+        row = {"dims": dims, "radix": r, "n": n}
+        try:
+            pl = pacg.Plan(dims, radix=r, precision=pacg.PREC_F64)
+            if zeros is None or zeros.size < n:
+                zeros = np.zeros(n)
+            dx = pacg.DeviceComplex(tuple(dims), np.float64)
+            dy = pacg.DeviceComplex(tuple(dims), np.float64)
+            for d in (dx, dy):
+                d.re.upload(zeros[:n])
+                d.im.upload(zeros[:n])
+            for name, op in (("ufft", pacg.OP_FWD), ("uifft", pacg.OP_UIFFT), ("permutation", pacg.OP_PERM)):
+                ms, _ = _timed_calls(pacg, pl.stream, lambda: None, lambda: pl.execute(op, dx, dy), reps)
+                row[name + "_s"] = ms * 1e-3
+                row[name + "_ns_per_element"] = ms * 1e6 / n
+            row["launches"] = pl.launches_per_convolve()
+            del pl, dx, dy
+        except Exception as exc:  # a shape without compiled kernels is reported, not fatal
+            row["error"] = str(exc)[:200]
+        pub_b = REF_PUBLISHED["butterflies"].get((dims_name, r), {}).get(n)
+        pub_p = REF_PUBLISHED["permutation"].get((dims_name, r), {}).get(n)
+        if pub_b:
+            row["reference_ufft_s"], row["reference_uifft_s"] = pub_b
+            if "ufft_s" in row:
+                row["speedup_ufft"] = pub_b[0] / row["ufft_s"]
+        if pub_p:
+            row["reference_permutation_s"] = pub_p
+        out.append(row)
+    return {"note": "FP64, device-resident, mean of %d calls after a warm-up; reference_* are the reference's "
+                    "published one-core seconds (forward_fft / backward_fft programs)" % reps, "rows": out}
+
+
 def bench_batched(pacg, synth, steps, warmup):
     """4096 FP32 pairs of lengths 2^U{8..14} (seed 4) in one persistent, graph-captured launch,
     against per-length plain FP64 launches (one batch-axis plan per length)."""
@@ -412,6 +637,10 @@ def bench_batched(pacg, synth, steps, warmup):
     for _ in range(steps):
         b.replay()
     ms_batch = t.stop() / steps
+    t.start()
+    for _ in range(steps):
+        b.convolve(dx, dk, dy)
+    ms_direct = t.stop() / steps
     ys = b.unpack(dy.numpy())
     # per-length plain FP64 launches
     ms64 = 0.0
@@ -445,6 +674,8 @@ def bench_batched(pacg, synth, steps, warmup):
             "gflops": flops / (ms_batch * 1e-3) / 1e9,
             "variants": {int(m): pacg.Batch.schedule(int(m), b.variant(int(m))) for m in range(8, 15)},
             "autotuned_lengths": changed,
+            "direct_launch_ms_per_batch": ms_direct,
+            "direct_launch_frac": b_alg / (ms_direct * 1e-3) / 1e9 / HBM_PEAK_GBS,
             "fp64_per_length_ms": ms64, "speedup_vs_fp64_per_length": ms64 / ms_batch,
             "rel_l2_vs_fp64": (num / den) ** 0.5 if den > 0 else 0.0, "max_abs_vs_fp64": maxerr}
 
@@ -454,7 +685,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--cpu-reps", type=int, default=4)
+    ap.add_argument("--cpu-reps", type=int, default=10,
+                    help="timed calls of the CPU baseline after one warm-up (the reference's default -r 10, "
+                         "bench1d_convolution.c:74)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-procs", type=int, default=16,
                     help="processes of the CPU throughput figure (0: skip); the box's CPU share is 16 per GPU")
@@ -465,6 +698,8 @@ def main():
                     help="--profile: the pair-launch plan instead of one axis per launch")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched uneven-length workload")
     ap.add_argument("--no-mfma", action="store_true", help="skip the 4096x256 MFMA workload")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the reference-semantics, FFT-based and per-element (ufft / uifft / permutation) sweeps")
     ap.add_argument("--profile-fp64-2d", action="store_true",
                     help="only run the 2D 2048^2 FP64 plan (radix 4, one radix-2 stage per axis), autotuned, "
                          "`steps` times (for rocprofv3)")
@@ -622,6 +857,7 @@ def main():
     b_alg64 = conv_alg_bytes(nl64, n, 16, 16)
     conv_gbs = b_alg / (ms32 * 1e-3) / 1e9
     flops = 2 * 5 * n * np.log2(n)  # two transforms (forward x, inverse), 5 N log2 N each
+    traffic, traffic_source = measured_traffic(dom[0])
 
     out = {
         "metric": "conv/s (3D 256^3 FP32 permutation-avoiding convolution)",
@@ -641,7 +877,7 @@ def main():
                                "FP32 butterflies, spectrum A^T k computed in FP64 on the GPU, stored %s" % a.spectrum,
                    "n": n, "schedules": SCHEDS32, "schedules_fp64": SCHEDS64, "launches_per_conv": len(prof32)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dom[0]),
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_source,
                      "kernel": dom[0], "kernel_ms": dom[1], "alg_bytes_per_launch": dom_bytes},
         "roofline_conv": {"alg_bytes": b_alg, "launches": nl32, "achieved_GBs": conv_gbs,
                           "frac": conv_gbs / HBM_PEAK_GBS},
@@ -677,13 +913,24 @@ def main():
         log("batched workload ...")
         out["batched"] = bench_batched(pacg, synth, a.steps, a.warmup)
         log("batched: %.3f ms per batch of 4096" % out["batched"]["ms_per_batch"])
+    if not a.no_extra:
+        log("reference semantics and FFT-based convolution at 256^3 ...")
+        out["reference_semantics_256cubed"] = bench_reference_semantics(pacg, x, k, a.steps)
+        log("FFT-based vs PA, 1D 2^24 ...")
+        out["fft_based_vs_pa_1d_2e24"] = bench_fft_based_1d(pacg, a.steps)
+        log("per-element ufft / uifft / permutation ...")
+        out["per_element"] = bench_per_element(pacg)
+    d_us, g_us = pacg.launch_floor(2000)
+    out["launch_floor"] = {"empty_kernel_direct_us": d_us, "empty_graph_replay_us": g_us,
+                           "note": "back-to-back launches of an empty kernel, and replays of a HIP graph holding "
+                                   "one; the 1D 2^16 line is bounded by these, not by its kernels"}
     log("hbm probe ...")
     probe = bench_hbm_probe(pacg)
     probe["fused_kernel_over_copy"] = out["roofline"]["achieved"] / probe["copy_GBs"]
     out["hbm_probe"] = probe
     if not a.no_cpu:
         log("cpu baseline ...")
-        out["cpu_baseline"] = cpu_baseline(x, k, a.cpu_reps)
+        out["cpu_baseline"] = cpu_baseline(a.cpu_reps)
         if a.cpu_procs > 0:
             log("cpu throughput, %d processes ..." % a.cpu_procs)
             out["cpu_baseline_throughput"] = cpu_throughput(min(a.cpu_procs, os.cpu_count() or 1))

@@ -24,7 +24,7 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "lib", "libpacg.so")
 
 PREC_F64, PREC_F32 = 0, 1
-OP_TRANS, OP_CONJ, OP_FWD, OP_FFT, OP_FFT2, OP_IFFT, OP_UIFFT = range(7)
+OP_TRANS, OP_CONJ, OP_FWD, OP_FFT, OP_FFT2, OP_IFFT, OP_UIFFT, OP_PERM = range(8)
 MAX_STAGES = 32
 
 _dp = C.POINTER(C.c_double)
@@ -95,6 +95,7 @@ def _load():
     L.pacg_plan_pin_launch.argtypes = [_vp, C.c_int, C.c_char_p]
     L.pacg_bw_probe.argtypes = [C.c_int, _vp, _vp, C.c_uint64, _vp]
     L.pacg_hbm_probe.argtypes = [C.c_int, _vp, _vp, C.c_uint64, C.c_int, _vp, C.POINTER(C.c_float)]
+    L.pacg_launch_floor.argtypes = [C.c_int, _vp, C.POINTER(C.c_float), C.POINTER(C.c_float)]
     L.pacg_dropin_release.argtypes = []
     L.pacg_dropin_release.restype = None
     L.pacg_desc_init.argtypes = [C.POINTER(_Desc), C.c_int, _ip, C.c_int, C.c_int]
@@ -586,6 +587,16 @@ def _conv(name, nd, with_rho):
 for _sfx, _nd in (("", 1), ("_2d", 2), ("_3d", 3)):
     globals()["permutation_avoiding_convolution" + _sfx] = _conv("permutation_avoiding_convolution" + _sfx, _nd, False)
     globals()["fft_based_convolution" + _sfx] = _conv("fft_based_convolution" + _sfx, _nd, True)
+
+
+def launch_floor(reps: int = 1000, stream=None):
+    """(direct, graph) microseconds per launch of an empty kernel: back-to-back launches, and
+    replays of a HIP graph holding one launch."""
+    d, g = C.c_float(0), C.c_float(0)
+    rc = lib.pacg_launch_floor(int(reps), stream, C.byref(d), C.byref(g))
+    if rc != 0:
+        raise PacgError("pacg_launch_floor failed (HIP error %d)" % rc)
+    return float(d.value), float(g.value)
 
 
 def release():

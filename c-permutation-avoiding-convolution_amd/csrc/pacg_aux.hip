@@ -267,6 +267,50 @@ int pacg_graph_destroy(void* exec) { return (int)hipGraphExecDestroy((hipGraphEx
 
 }  // extern "C"
 
+namespace {
+__global__ void empty_kernel() {}
+}  // namespace
+
+extern "C" int pacg_launch_floor(int reps, void* stream, float* direct_us, float* graph_us) {
+  if (reps < 1 || !direct_us || !graph_us) return (int)hipErrorInvalidValue;
+  hipStream_t s = (hipStream_t)stream, own = nullptr;
+  hipError_t e = hipSuccess;
+  if (!s) {
+    if ((e = hipStreamCreateWithFlags(&own, hipStreamNonBlocking)) != hipSuccess) return (int)e;
+    s = own;
+  }
+  hipEvent_t a = nullptr, b = nullptr;
+  hipGraph_t g = nullptr;
+  hipGraphExec_t x = nullptr;
+  float ms = 0.0f;
+  if ((e = hipEventCreate(&a)) != hipSuccess || (e = hipEventCreate(&b)) != hipSuccess) goto out;
+  for (int i = 0; i < 10; ++i) hipLaunchKernelGGL(empty_kernel, dim3(1), dim3(64), 0, s);
+  if ((e = hipStreamSynchronize(s)) != hipSuccess) goto out;
+  hipEventRecord(a, s);
+  for (int i = 0; i < reps; ++i) hipLaunchKernelGGL(empty_kernel, dim3(1), dim3(64), 0, s);
+  hipEventRecord(b, s);
+  if ((e = hipEventSynchronize(b)) != hipSuccess || (e = hipEventElapsedTime(&ms, a, b)) != hipSuccess) goto out;
+  *direct_us = 1000.0f * ms / (float)reps;
+  if ((e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal)) != hipSuccess) goto out;
+  hipLaunchKernelGGL(empty_kernel, dim3(1), dim3(64), 0, s);
+  if ((e = hipStreamEndCapture(s, &g)) != hipSuccess) goto out;
+  if ((e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0)) != hipSuccess) goto out;
+  for (int i = 0; i < 10; ++i) hipGraphLaunch(x, s);
+  if ((e = hipStreamSynchronize(s)) != hipSuccess) goto out;
+  hipEventRecord(a, s);
+  for (int i = 0; i < reps; ++i) hipGraphLaunch(x, s);
+  hipEventRecord(b, s);
+  if ((e = hipEventSynchronize(b)) != hipSuccess || (e = hipEventElapsedTime(&ms, a, b)) != hipSuccess) goto out;
+  *graph_us = 1000.0f * ms / (float)reps;
+out:
+  if (x) hipGraphExecDestroy(x);
+  if (g) hipGraphDestroy(g);
+  if (a) hipEventDestroy(a);
+  if (b) hipEventDestroy(b);
+  if (own) hipStreamDestroy(own);
+  return (int)e;
+}
+
 // Scratch of the allocation-free entry point (pacg_convolve_fft_based_noalloc):
 // a fixed array in the code object, one copy on every GPU the library runs
 // on, held until the process ends.  Its name is a host-side stand-in for a

@@ -1016,6 +1016,15 @@ static int execute_with(pacg_plan* pl, int op, const void* ir, const void* ii, v
       if ((rc = run_all(pl, prec, PACG_K_DIF, ir, ii, sr, si, 1.0))) return rc;
       HIPCHK(pacg_permute(sr, si, orr, oi, prec, &pd, 1, pl->stream), "permute");
       return PACG_OK;
+    case PACG_OP_PERM:
+      perm_desc(pl, &pd);
+      if (ir != orr || ii != oi) {
+        HIPCHK(pacg_permute(ir, ii, orr, oi, prec, &pd, 0, pl->stream), "permute");
+        return PACG_OK;
+      }
+      HIPCHK(pacg_permute(ir, ii, sr, si, prec, &pd, 0, pl->stream), "permute");
+      HIPCHK(pacg_convert(sr, si, prec, orr, oi, prec, (uint64_t)pl->n, 1.0, pl->stream), "copy");
+      return PACG_OK;
     default:
       return pacg_fail(PACG_ERR_ARG, "pacg_execute: unknown op %d", op);
   }
@@ -1024,7 +1033,9 @@ static int execute_with(pacg_plan* pl, int op, const void* ir, const void* ii, v
 int pacg_execute(pacg_plan* pl, int op, const void* ir, const void* ii, void* orr, void* oi) {
   if (!pl || !ir || !ii || !orr || !oi) return pacg_fail(PACG_ERR_ARG, "pacg_execute: NULL argument");
   int rc;
-  if ((op == PACG_OP_FFT || op == PACG_OP_IFFT || op == PACG_OP_FFT2) && (rc = ensure_scratch(pl))) return rc;
+  if ((op == PACG_OP_FFT || op == PACG_OP_IFFT || op == PACG_OP_FFT2 || (op == PACG_OP_PERM && ir == orr)) &&
+      (rc = ensure_scratch(pl)))
+    return rc;
   return execute_with(pl, op, ir, ii, orr, oi, pl->scr_re, pl->scr_im);
 }
 

@@ -49,7 +49,10 @@ enum {
   PACG_OP_FFT = 3,    /* A P            (fft: natural-order DFT)                 */
   PACG_OP_FFT2 = 4,   /* P^-1 A^T       (fft2: natural-order DFT)                */
   PACG_OP_IFFT = 5,   /* conj(A) P / n  (ifft)                                   */
-  PACG_OP_UIFFT = 6   /* conj(A) / n    (uifft)                                  */
+  PACG_OP_UIFFT = 6,  /* conj(A) / n    (uifft)                                  */
+  PACG_OP_PERM = 7    /* P: x[p] <- x[rho(p)], the digit-reversal permutation of
+                         apply_permutation_split (index_reversals.c:85-94); in
+                         place through the plan's scratch                         */
 };
 
 #define PACG_MAX_STAGES 32
@@ -189,6 +192,12 @@ void pacg_dropin_release(void);
  * of one launch.  bytes must be a positive multiple of 16 and src / dst device
  * buffers of at least that size (src unused by mode 1, dst by mode 0). */
 int pacg_hbm_probe(int mode, void* dst, const void* src, uint64_t bytes, int reps, void* stream, float* ms);
+
+/* Launch floor: mean time per launch, in microseconds, of an empty kernel
+ * launched `reps` times back to back on `stream` (NULL: a private stream) and
+ * of a HIP graph holding one such launch replayed `reps` times.  Measurement
+ * aid; no counterpart in the reference. */
+int pacg_launch_floor(int reps, void* stream, float* direct_us, float* graph_us);
 
 /* Wait for the plan stream. */
 int pacg_sync(pacg_plan* plan);

@@ -41,3 +41,28 @@ def test_cpu_baseline_workloads_report_every_workload():
             assert e["gpu_speedup"] is None
         else:
             assert e["gpu_speedup"] == pytest.approx(sec * 1e3 / gpu[name])
+
+
+def test_cpu_baseline_runs_pinned_to_one_core():
+    """The CPU baseline runs in a child process pinned to one core (the reference's taskset -c 0) and
+    reports the host's CPU model, core counts and the reference library's compile flags."""
+    import bench
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from oracle import RefLib
+
+    if not RefLib.available():
+        pytest.skip("oracle/_ref not built")
+    r = bench.cpu_baseline(1)
+    assert len(r["pinned_cores"]) == 1 and r["cores"] == 1
+    assert r["warmup_calls"] == 1 and r["timed_calls"] == 1 and r["value"] > 0
+    assert r["kind"] == "reference" and "-O3" in r["compile_flags"] and "-ffast-math" in r["compile_flags"]
+    assert r["host_cores"] >= 1 and r["cpu_model"]
+
+
+def test_traffic_figure_names_its_source():
+    import bench
+
+    t, src = bench.measured_traffic("f32_fused_s_4x8x8_h_l32")
+    assert t and t > 0 and "profiles/traffic_3d256_fp32.json" in src
+    t, src = bench.measured_traffic("no_such_kernel")
+    assert t is None and "no entry" in src

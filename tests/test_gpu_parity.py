@@ -102,6 +102,30 @@ def test_pair_and_fft_based_fp64(gpu, oracle, dims, scheds):
     assert rel_l2(dh.numpy(), fh_ref) < TOL64
 
 
+@pytest.mark.parametrize("dims,scheds", CASES)
+@pytest.mark.parametrize("prec", [0, 1])
+def test_permutation_op(gpu, oracle, dims, scheds, prec):
+    """PACG_OP_PERM: x[p] <- x[rho(p)] with the oracle's digit reversal, exactly (data movement only),
+    into another array and in place (through the plan's scratch)."""
+    x = rnd(11, dims)
+    dt = np.float64 if prec == 0 else np.float32
+    rho = oracle.rho(list(dims), scheds)
+    want = x.astype(np.complex128 if prec == 0 else np.complex64).ravel()[rho].reshape(dims)
+    pl = gpu.Plan(dims, schedules=scheds, precision=prec)
+    dx = gpu.DeviceComplex.from_numpy(x, dt)
+    dy = pl.alloc()
+    pl.execute(gpu.OP_PERM, dx, dy)
+    assert np.array_equal(dy.numpy(), want)
+    pl.execute(gpu.OP_PERM, dx)  # in place
+    assert np.array_equal(dx.numpy(), want)
+    pl.close()
+
+
+def test_launch_floor(gpu):
+    d_us, g_us = gpu.launch_floor(200)
+    assert 0.0 < d_us < 1000.0 and 0.0 < g_us < 1000.0
+
+
 @pytest.mark.parametrize("dims,scheds", [c for c in CASES if np.prod(c[0]) >= 8])
 @pytest.mark.parametrize("sprec", [0, 1])
 def test_convolve_fp32(gpu, oracle, dims, scheds, sprec):
