@@ -704,7 +704,8 @@ def main():
                     help="only run the 2D 2048^2 FP64 plan (radix 4, one radix-2 stage per axis), autotuned, "
                          "`steps` times (for rocprofv3)")
     ap.add_argument("--profile-mfma", action="store_true",
-                    help="only run the 4096x256 MFMA convolution `steps` times (for rocprofv3 counters)")
+                    help="only run the 4096x256 convolution `steps` times with the MFMA plan, then with the VALU "
+                         "plan (for rocprofv3 counters)")
     ap.add_argument("--profile-batched", action="store_true",
                     help="only run the batched workload (graph replay) `steps` times (for rocprofv3)")
     ap.add_argument("--profile", action="store_true",
@@ -750,17 +751,21 @@ def main():
 
     if a.profile_mfma:
         x4, k4 = synth.workload_inputs("2d_4096x256")
-        pl = pacg.Plan(x4.shape, schedules=[[8, 8, 8, 8], [8, 8, 4]], precision=pacg.PREC_F32,
-                       spectrum_precision=pacg.PREC_F32, mfma=True)
-        pl.set_kernel(k4)
         dx = pacg.DeviceComplex.from_numpy(x4, np.float32)
-        dy = pl.alloc()
-        pl.autotune(dx, dy, 3)  # the tiles the bench line reports
-        pl.sync()
-        for _ in range(a.steps):
-            pl.convolve(dx, dy)
-        pl.sync()
-        print(json.dumps({"profile_mfma": True, "steps": a.steps, "plan": pl.describe()}))
+        plans = {}
+        for name, mf in (("mfma", True), ("valu", False)):  # the MFMA plan, then the VALU plan beside it
+            pl = pacg.Plan(x4.shape, schedules=[[8, 8, 8, 8], [8, 8, 4]], precision=pacg.PREC_F32,
+                           spectrum_precision=pacg.PREC_F32, mfma=mf)
+            pl.set_kernel(k4)
+            dy = pl.alloc()
+            pl.autotune(dx, dy, 3)  # the tiles the bench line reports
+            pl.sync()
+            for _ in range(a.steps):
+                pl.convolve(dx, dy)
+            pl.sync()
+            plans[name] = pl.describe()
+            del pl, dy
+        print(json.dumps({"profile_mfma": True, "steps": a.steps, "plans": plans}))
         return
 
     t0 = time.time()

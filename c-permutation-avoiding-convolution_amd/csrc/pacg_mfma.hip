@@ -77,6 +77,9 @@ __device__ __forceinline__ void f8_mfma(const Mat& m, float x[4]) {
   mx = fmaxf(mx, __shfl_xor(mx, 32));
   int ex = 0;
   if (mx > 0.f) ex = ((int)((__float_as_uint(mx) >> 23) & 0xff)) - 126;  // mx = f 2^ex, f in [0.5, 1)
+  // at most 2^127: mx < 2^128 keeps the scaled values below 2 (inside FP16), and 2^128 would not
+  // be a finite FP32 factor to scale back with
+  ex = ex > 127 ? 127 : ex;
   const float down = ldexpf(1.0f, -ex), up = ldexpf(1.0f, ex);
   half4_t bh, bl;
   for (int v = 0; v < 4; ++v) {

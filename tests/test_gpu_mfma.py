@@ -74,6 +74,25 @@ def test_mfma_block_scaling_handles_wide_range(gpu, oracle):
     assert rel_l2(dx.numpy(), ref) < TOL32
 
 
+@pytest.mark.parametrize("scale", [1.5 * 2.0 ** 127, 2.0 ** 127, 1.0])
+def test_mfma_block_scaling_near_float_max(gpu, scale):
+    """A delta of magnitude up to near FLT_MAX: every butterfly of A^T sees one nonzero input, so
+    the exact result is that value at every position; the block scale must stay a finite FP32
+    factor (2^128 is not)."""
+    n = 512
+    x = np.zeros(n, complex)
+    x[0] = scale * (1 - 0.5j)
+    pl = gpu.Plan((n,), schedules=[[8, 8, 8]], precision=gpu.PREC_F32, mfma=True)
+    assert "mfma" in pl.describe()
+    dx = gpu.DeviceComplex.from_numpy(x, np.float32)
+    pl.execute(gpu.OP_TRANS, dx)
+    y = dx.numpy()
+    want = np.full(n, np.complex64(x[0]), dtype=complex)
+    assert np.all(np.isfinite(y))
+    assert rel_l2(y, want) < 2e-6, rel_l2(y, want)
+    pl.close()
+
+
 def test_anisotropic_4096x256_full_size(gpu, synth):
     """Config 4: 4096 x 256, U(-1,1), seed 3, radix 8-8-8-8 and 8-8-4, MFMA radix-8 stages."""
     x, k = synth.workload_inputs("2d_4096x256")
