@@ -323,7 +323,7 @@ static void dropin(int op, int nd, const int* dims, int r, double* re, double* i
   pacg_plan* pl = s->plan;
   void* st = pacg_plan_stream(pl);
   long long n = pacg_plan_size(pl);
-  int rc = 0;
+  int rc = 0, ok = 0;
   rc |= up(s, s->xr, re, 0);
   rc |= up(s, s->xi, im, 1);
   if (op == OPK_PA || op == OPK_FB) {
@@ -381,7 +381,11 @@ static void dropin(int op, int nd, const int* dims, int r, double* re, double* i
     goto done;
   }
   {
-    /* stage results through the slot so a failure leaves the caller's arrays untouched */
+    /* Arrays up to 1 MiB come back through the slot's staging buffer and are
+     * copied out only once every copy has succeeded, so a failure leaves the
+     * caller's arrays untouched.  Larger arrays come back through the pinned
+     * ring straight into the caller's arrays: a HIP failure during those copies
+     * can leave them partly written. */
     int e = 0;
     e |= pacg_stream_sync(st);
     if (e) {
@@ -405,9 +409,13 @@ static void dropin(int op, int nd, const int* dims, int r, double* re, double* i
         finish_down(s, him, 3);
       }
       pacg_clear_error();
+      ok = 1;
     }
   }
 done:
+  /* on a failure, wait for the slot's stream before the next call may reuse the
+   * staging buffers and the ring: no copy may still touch them or the caller's arrays */
+  if (!ok) (void)pacg_stream_sync(st);
   pthread_mutex_unlock(&g_mu);
 }
 

@@ -20,6 +20,9 @@
  *   - rho arrays are applied as the gather x[j] <- x[rho[j]], which is the
  *     reference's swap loop (index_reversals.c:85-94) for every rho the
  *     reference's precompute functions return (they are involutions).
+ * A HIP failure (pacg_last_error() set after the call) leaves arrays of up to
+ * 1 MiB untouched; larger arrays are copied back in 8 MiB chunks and can be
+ * left partly written.
  */
 #ifndef PACG_FFT_H
 #define PACG_FFT_H
