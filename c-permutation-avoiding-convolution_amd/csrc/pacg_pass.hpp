@@ -665,6 +665,18 @@ __global__ void __launch_bounds__(LINES* SH::TPL)
     else
       store_impl(PH_, std::true_type{}, scale);
   };
+  // Contiguous lines of at most 64 threads lie inside one wavefront (tl is
+  // tid % TPL, TPL a power of two), and every slot an exchange writes or reads
+  // belongs to the thread's own line: the wave only has to wait for its own
+  // LDS writes before reading (LDS operations of one wave complete in order),
+  // not for the other waves of the workgroup.
+  constexpr bool WAVE_X = LAYOUT == PACG_L_CONTIG && TPL <= 64;
+  auto xsync = [&]() PACG_INL {
+    if constexpr (WAVE_X)
+      asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+    else
+      __syncthreads();
+  };
   auto exchange = [&](auto FROM_, auto TO_) PACG_INL {
     constexpr int FROM = decltype(FROM_)::value;
     constexpr int TO = decltype(TO_)::value;
@@ -672,7 +684,7 @@ __global__ void __launch_bounds__(LINES* SH::TPL)
     constexpr int RT = 1 << SH::lgR(TO);
     if constexpr (SPLIT) {
       auto move = [&](T* v) PACG_INL {
-        __syncthreads();
+        xsync();
         sfor<E / RF>([&](auto c_) PACG_INL {
           constexpr int c = decltype(c_)::value;
           sfor<RF>([&](auto r_) PACG_INL {
@@ -680,7 +692,7 @@ __global__ void __launch_bounds__(LINES* SH::TPL)
             xr[lds_slot(FROM_, c_, r_)] = v[c * RF + r];
           });
         });
-        __syncthreads();
+        xsync();
         sfor<E / RT>([&](auto c_) PACG_INL {
           constexpr int c = decltype(c_)::value;
           sfor<RT>([&](auto r_) PACG_INL {
@@ -694,7 +706,7 @@ __global__ void __launch_bounds__(LINES* SH::TPL)
       return;
     }
     if constexpr (PAIRS) {
-      __syncthreads();
+      xsync();
       sfor<E / RF>([&](auto c_) PACG_INL {
         constexpr int c = decltype(c_)::value;
         sfor<RF>([&](auto r_) PACG_INL {
@@ -705,7 +717,7 @@ __global__ void __launch_bounds__(LINES* SH::TPL)
           xv[lds_slot(FROM_, c_, r_)] = w;
         });
       });
-      __syncthreads();
+      xsync();
       sfor<E / RT>([&](auto c_) PACG_INL {
         constexpr int c = decltype(c_)::value;
         sfor<RT>([&](auto r_) PACG_INL {
@@ -717,7 +729,7 @@ __global__ void __launch_bounds__(LINES* SH::TPL)
       });
       return;
     }
-    __syncthreads();
+    xsync();
     sfor<E / RF>([&](auto c_) PACG_INL {
       constexpr int c = decltype(c_)::value;
       sfor<RF>([&](auto r_) PACG_INL {
@@ -727,7 +739,7 @@ __global__ void __launch_bounds__(LINES* SH::TPL)
         xi[li] = vi[c * RF + r];
       });
     });
-    __syncthreads();
+    xsync();
     sfor<E / RT>([&](auto c_) PACG_INL {
       constexpr int c = decltype(c_)::value;
       sfor<RT>([&](auto r_) PACG_INL {
