@@ -19,10 +19,19 @@ SHAPES = [[2] * k for k in range(1, 9)] + [[4] * k for k in range(1, 5)] + [[8] 
 # ... and the mixed runs of the mixed-radix schedules the planner uses:
 # 256 = 4*8*8 and 8*8*4, 2048 = 4^5 * 2 split as (4^4)(4, 2), 32 = 4*8 / 8*4
 SHAPES += [[4, 8, 8], [8, 8, 4], [4, 2], [2, 4], [4, 8], [8, 4], [8, 2], [2, 8]]
-# whole 2048-point axes in one pass, contiguous lines only (a strided tile of
-# 2048-point lines would not leave room for enough lines in LDS): 4^5 * 2,
-# 2^11 (the reference's radix for 2048) and 8^3 * 4 / 4 * 8^3
+# whole 2048-point axes in one pass on contiguous lines: 4^5 * 2, 2^11 (the
+# reference's radix for 2048) and 8^3 * 4 / 4 * 8^3
 LONG_SHAPES = [[4, 4, 4, 4, 4, 2], [2] * 11, [8, 8, 8, 4], [4, 8, 8, 8]]
+# ... and on strided lines in FP64 (the strided axis of a 2048^2 array in one
+# pass: 3 round trips per 2D convolution instead of 5).  A tile is 2, 4 or 8
+# lines of 2048 points (32 KiB each in FP64), so few lines fit in LDS: points
+# per thread -> (lines per tile, rank) compiled, rank 0 the default.  16 points
+# on 8 lines (64-byte runs of each row) are fastest at 2048^2 (about 37 against
+# 50 us with 8 points on 2 lines); no fused pass with 16 points per thread
+# (data and spectrum would spill), so the fused pass defaults to 8 on 2 lines.
+LONG_STRIDED = [[4, 4, 4, 4, 4, 2]]
+LONG_STRIDED_TILES = {16: [(8, 0), (4, 1)], 8: [(2, 2), (4, 3)]}
+LONG_STRIDED_FUSED = [(2, 0), (4, 1)]
 
 E_TARGET = 16
 
@@ -127,10 +136,19 @@ def emit(prec: str, outdir: str):
         P, E, tpl, sl, nl = geometry(shape, prec, et)
         shname = "Shape<%d, 0x%Xu, %d, 0x%Xu>" % (len(shape), sc, npph, pc)
         sname = "x".join(str(r) for r in shape) + suffix
+        long_s = prec == "double" and shape in LONG_STRIDED
         for kind in KINDS:
-            for layout in (LAYOUTS if shape in SHAPES else ["PACG_L_CONTIG"]):
+            for layout in (LAYOUTS if shape in SHAPES or long_s else ["PACG_L_CONTIG"]):
                 specs = spec_types if kind == "PACG_K_FUSED" else [T]
-                for rank, ln in enumerate(tile_options(shape, prec, layout, et), start=rank0):
+                if shape in SHAPES or layout == "PACG_L_CONTIG":
+                    tiles = [(ln, rank0 + i) for i, ln in enumerate(tile_options(shape, prec, layout, et))]
+                elif kind == "PACG_K_FUSED":
+                    if E > 8:
+                        continue
+                    tiles = LONG_STRIDED_FUSED
+                else:
+                    tiles = LONG_STRIDED_TILES[E]
+                for ln, rank in tiles:
                     for ts in specs:
                         fn = "launch_pass<%s, %s, %s, %s, %s, %d>" % (T, ts, shname, kind, layout, ln)
                         sp = "PACG_F64" if ts == "double" else "PACG_F32"

@@ -297,9 +297,10 @@ __device__ __forceinline__ void phase_stages(T* vr, T* vi) {
 }
 
 // Phase-external twiddle w_{R lambda}^{mu k(r)}; tw(idx, wr, wi) returns w_P^idx.
-// SERIAL: finish one group before the next group's table reads are issued
-// (keeps register pressure down where many groups share a thread).
-template <typename T, class SH, int PH, bool CJ, typename TW, bool SERIAL = false>
+// SERIAL 1: finish one group before the next group's table reads are issued;
+// 2: one twiddle at a time (keeps register pressure down where each twiddle
+// takes two table reads and a product, as for FP64 lines of 16-point phases).
+template <typename T, class SH, int PH, bool CJ, int SERIAL = 0, typename TW>
 __device__ __forceinline__ void phase_twiddle_f(T* vr, T* vi, int tl, TW&& tw) {
   constexpr int lo = SH::plo(PH);
   if constexpr (lo > 0) {
@@ -320,9 +321,10 @@ __device__ __forceinline__ void phase_twiddle_f(T* vr, T* vi, int tl, TW&& tw) {
           tw(idx, wr, wi);
           if constexpr (CJ) wi = -wi;
           cmul<T>(vr[c * R + r], vi[c * R + r], wr, wi);
+          if constexpr (SERIAL == 2) __builtin_amdgcn_sched_barrier(0);
         }
       });
-      if constexpr (SERIAL) __builtin_amdgcn_sched_barrier(0);
+      if constexpr (SERIAL == 1) __builtin_amdgcn_sched_barrier(0);
     });
   }
 }
@@ -427,13 +429,37 @@ __device__ __forceinline__ int row_flip(int s) {
     return __builtin_parity((unsigned)s & 0x14u) | (__builtin_parity((unsigned)s & 0x28u) << 1);
 }
 
-enum { LM_PAD = 0, LM_ROWS = 1, LM_ROWSWZ = 2, LM_LINESWZ = 3 };
+enum { LM_PAD = 0, LM_ROWS = 1, LM_ROWSWZ = 2, LM_LINESWZ = 3, LM_ROWXOR = 4 };
+
+// Row of point s in a strided FP64 tile of long lines ([P][LINES], no
+// padding): the low NB bits of s are XOR-ed with every NB-bit group of s above
+// them.  In the bottom phases consecutive threads hold points R rows apart;
+// the fold spreads them over distinct rows modulo 2^NB, and for every phase
+// layout of the 2048-point shapes the 32 lanes of a half-wave meet each of
+// the 32 banks equally often (NB = 3 up to 4 lines, 4 for 8 lines).  Linear
+// over GF(2), like the other layouts.
+template <int NB, int LGP>
+__device__ __forceinline__ constexpr int row_fold(int s) {
+  int h = 0;
+  for (int k = NB; k < LGP; k += NB) h ^= s >> k;  // a fixed count: no loop at run time
+  return s ^ (h & ((1 << NB) - 1));
+}
 
 // ---------------------------------------------------------------------------
 // the pass kernel
 // ---------------------------------------------------------------------------
+// Minimum waves per SIMD the register allocation must allow.  Strided FP64
+// tiles of long lines hold 2-8 lines per workgroup of 256-1024 threads, and LDS
+// leaves room for 4 waves per SIMD (two 512-thread or four 256-thread
+// workgroups per CU): without the bound the allocator takes up to 254 VGPRs
+// for the smaller workgroups (2 waves per SIMD).  Other tiles: no bound.
+template <typename T, class SH, int LAYOUT>
+constexpr int pass_min_waves() {
+  return (sizeof(T) == 8 && LAYOUT == PACG_L_STRIDED && SH::P >= 2048) ? 4 : 1;
+}
+
 template <typename T, typename TS, class SH, int KIND, int LAYOUT, int LINES>
-__global__ void __launch_bounds__(LINES* SH::TPL)
+__global__ void __launch_bounds__(LINES* SH::TPL) __attribute__((amdgpu_waves_per_eu(pass_min_waves<T, SH, LAYOUT>())))
     pass_kernel(const pacg_pass_args_t a) {
   static_assert(SH::valid(), "invalid pass shape");
   constexpr int P = SH::P, E = SH::E, TPL = SH::TPL, NP = SH::NP;
@@ -446,16 +472,21 @@ __global__ void __launch_bounds__(LINES* SH::TPL)
   //              bits of the row XOR-ed with parities of the point index (row_flip);
   //   LM_LINESWZ contiguous, 256 points or more at most 8 per thread, or 2048
   //              points: [LINES][P] with lds_swz inside each line.
-  // Other tiles and FP64 keep LM_PAD: strided [P][LINES+1], contiguous
-  // [LINES][P + P/8 + 1] with one pad slot per 8 points.
+  // FP64 strided tiles of lines of 2048+ points (a few lines wide, so that a
+  // tile fits in LDS at all) use LM_ROWXOR: [P][LINES] with the row folded by
+  // row_fold, no padding.  Other tiles and FP64 keep LM_PAD: strided
+  // [P][LINES+1], contiguous [LINES][P + P/8 + 1] with one pad slot per 8 points.
   constexpr bool F32 = sizeof(T) == 4;
-  constexpr int LMODE = !F32                        ? LM_PAD
+  constexpr bool LONG_S = !F32 && LAYOUT == PACG_L_STRIDED && P >= 2048;
+  constexpr int LMODE = LONG_S                      ? LM_ROWXOR
+                        : !F32                      ? LM_PAD
                         : LAYOUT == PACG_L_STRIDED ? (LINES >= 32                  ? LM_ROWS
                                                       : (LINES == 16 || LINES == 8) ? LM_ROWSWZ
                                                                                     : LM_PAD)
                         : ((E <= 8 && P >= 256) || P >= 2048) ? LM_LINESWZ
                                                               : LM_PAD;
-  constexpr int LDJ = (LMODE == LM_ROWS || LMODE == LM_ROWSWZ) ? LINES : LINES + 1;
+  constexpr int XNB = LINES >= 8 ? 4 : 3;  // fold width of LM_ROWXOR
+  constexpr int LDJ = (LMODE == LM_ROWS || LMODE == LM_ROWSWZ || LMODE == LM_ROWXOR) ? LINES : LINES + 1;
   constexpr int LDP = LMODE == LM_LINESWZ ? P : P + P / 8 + 1;
   constexpr int LDS_N = (LAYOUT == PACG_L_STRIDED) ? P * LDJ : LINES * LDP;
   constexpr bool NEED_X = NP > 1;
@@ -465,8 +496,10 @@ __global__ void __launch_bounds__(LINES* SH::TPL)
   // per SIMD, where LDS and not registers bounded the occupancy; 44.5 -> 37.6
   // us per pass at 2048^2).  With 8 points per thread (the default for these
   // lines) the paired exchange below is faster (about 31 against 33 us), and
-  // tiles bounded by registers gain nothing from the split.
-  constexpr bool SPLIT = !F32 && LAYOUT == PACG_L_CONTIG && P >= 2048 && E > 8;
+  // tiles bounded by registers gain nothing from the split.  Strided FP64
+  // tiles of long lines split too, except 2-line tiles of 8 points per thread
+  // (64 KiB paired: two 512-thread workgroups per CU either way).
+  constexpr bool SPLIT = (!F32 && LAYOUT == PACG_L_CONTIG && P >= 2048 && E > 8) || (LONG_S && (E > 8 || LINES > 2));
   // The tiles hold (re, im) pairs: one 8- or 16-byte LDS access per point and
   // per twiddle instead of two (the slot indices of the layouts above are
   // unchanged).  At 256^3 FP32 the fused pass goes from 79.5 to 76 us, with
@@ -511,6 +544,9 @@ __global__ void __launch_bounds__(LINES* SH::TPL)
     line_base = line * (uint32_t)P;
   }
   const uint32_t stride = (LAYOUT == PACG_L_STRIDED) ? S : 1u;
+  // Lines past the end (in the last tile only) read line 0 and store nothing:
+  // loads need no predicate, so no branch per element and no zero fill.
+  const uint32_t ld_base = valid ? line_base : 0u;
 
   T vr[E], vi[E];
 
@@ -531,8 +567,11 @@ __global__ void __launch_bounds__(LINES* SH::TPL)
   auto ptw = [&](auto PH_, auto CJ_) PACG_INL {
     constexpr int PH = decltype(PH_)::value;
     constexpr bool CJ = decltype(CJ_)::value;
+    // FP64 strided tiles of long lines with 16-point phases: one twiddle at a
+    // time (with all 30 table reads of a phase hoisted they spill to scratch)
+    constexpr int SER = (LONG_S && E >= 16) ? 2 : 0;
     if constexpr (PAIRS && TW2) {
-      phase_twiddle_f<T, SH, PH, CJ>(vr, vi, tl, [&](int idx, T& wr, T& wi) PACG_INL {
+      phase_twiddle_f<T, SH, PH, CJ, SER>(vr, vi, tl, [&](int idx, T& wr, T& wi) PACG_INL {
         const V2T h = twv[idx >> 6], l = twv[P / 64 + (idx & 63)];
         wr = h.x * l.x - h.y * l.y;
         wi = h.x * l.y + h.y * l.x;
@@ -544,7 +583,7 @@ __global__ void __launch_bounds__(LINES* SH::TPL)
         wi = w.y;
       });
     } else if constexpr (TW2) {
-      phase_twiddle_f<T, SH, PH, CJ>(vr, vi, tl, [&](int idx, T& wr, T& wi) PACG_INL {
+      phase_twiddle_f<T, SH, PH, CJ, SER>(vr, vi, tl, [&](int idx, T& wr, T& wi) PACG_INL {
         const T hr = twr[idx >> 6], hi = twi[idx >> 6];
         const T lr = twr[P / 64 + (idx & 63)], li = twi[P / 64 + (idx & 63)];
         wr = hr * lr - hi * li;
@@ -575,6 +614,9 @@ __global__ void __launch_bounds__(LINES* SH::TPL)
       return ((((st ^ row_flip<LINES>(st)) ^ pat) << LGL) | ll) + rest;
     } else if constexpr (LMODE == LM_LINESWZ) {
       return ((ll * P) | lds_swz(st)) ^ lds_swz(sc);
+    } else if constexpr (LMODE == LM_ROWXOR) {
+      // row_fold is linear and ll < LINES: the constant part is one XOR
+      return (row_fold<XNB, SH::LGP>(st) * LINES + ll) ^ (row_fold<XNB, SH::LGP>(sc) * LINES);
     } else if constexpr (LAYOUT == PACG_L_STRIDED) {
       return (st | sc) * LDJ + ll;
     } else {
@@ -595,11 +637,11 @@ __global__ void __launch_bounds__(LINES* SH::TPL)
       if (a.vec16) {
         sfor<E / R>([&](auto c_) PACG_INL {
           constexpr int c = decltype(c_)::value;
-          const uint32_t off = line_base + (uint32_t)pos<SH, 0>(tl, c, 0);
+          const uint32_t off = ld_base + (uint32_t)pos<SH, 0>(tl, c, 0);
           sfor<R / VW>([&](auto v_) PACG_INL {
             constexpr int v = decltype(v_)::value;
-            V4 xr4 = valid ? reinterpret_cast<const V4*>(br + off)[v] : V4{};
-            V4 xi4 = valid ? reinterpret_cast<const V4*>(bi + off)[v] : V4{};
+            const V4 xr4 = reinterpret_cast<const V4*>(br + off)[v];
+            const V4 xi4 = reinterpret_cast<const V4*>(bi + off)[v];
             Vec4<T>::split(xr4, &vr[c * R + v * VW]);
             Vec4<T>::split(xi4, &vi[c * R + v * VW]);
           });
@@ -612,9 +654,9 @@ __global__ void __launch_bounds__(LINES* SH::TPL)
       sfor<R>([&](auto r_) PACG_INL {
         constexpr int r = decltype(r_)::value;
         const int s = pos<SH, PH>(tl, c, r);
-        const uint32_t off = line_base + (uint32_t)s * stride;
-        vr[c * R + r] = valid ? br[off] : T(0);
-        vi[c * R + r] = valid ? bi[off] : T(0);
+        const uint32_t off = ld_base + (uint32_t)s * stride;
+        vr[c * R + r] = br[off];
+        vi[c * R + r] = bi[off];
       });
     });
   };
@@ -622,6 +664,8 @@ __global__ void __launch_bounds__(LINES* SH::TPL)
     constexpr int PH = decltype(PH_)::value;
     constexpr bool SCALED = decltype(SCALED_)::value;
     constexpr int R = 1 << SH::lgR(PH);
+    // one branch around all the stores of the phase (not one per element)
+    if (!valid) return;
     if constexpr (PH == 0 && VEC0) {
       if (a.vec16) {
         sfor<E / R>([&](auto c_) PACG_INL {
@@ -635,10 +679,8 @@ __global__ void __launch_bounds__(LINES* SH::TPL)
               tr[u] = SCALED ? vr[c * R + v * VW + u] * scale : vr[c * R + v * VW + u];
               ti[u] = SCALED ? vi[c * R + v * VW + u] * scale : vi[c * R + v * VW + u];
             }
-            if (valid) {
-              reinterpret_cast<V4*>(outr + off)[v] = Vec4<T>::join(tr);
-              reinterpret_cast<V4*>(outi + off)[v] = Vec4<T>::join(ti);
-            }
+            reinterpret_cast<V4*>(outr + off)[v] = Vec4<T>::join(tr);
+            reinterpret_cast<V4*>(outi + off)[v] = Vec4<T>::join(ti);
           });
         });
         return;
@@ -650,10 +692,8 @@ __global__ void __launch_bounds__(LINES* SH::TPL)
         constexpr int r = decltype(r_)::value;
         const int s = pos<SH, PH>(tl, c, r);
         const uint32_t off = line_base + (uint32_t)s * stride;
-        if (valid) {
-          outr[off] = SCALED ? vr[c * R + r] * scale : vr[c * R + r];
-          outi[off] = SCALED ? vi[c * R + r] * scale : vi[c * R + r];
-        }
+        outr[off] = SCALED ? vr[c * R + r] * scale : vr[c * R + r];
+        outi[off] = SCALED ? vi[c * R + r] * scale : vi[c * R + r];
       });
     });
   };
@@ -770,6 +810,9 @@ __global__ void __launch_bounds__(LINES* SH::TPL)
         T wi = h.x * l.y + h.y * l.x;
         if (cj) wi = -wi;
         cmul<T>(vr[c * R0 + r], vi[c * R0 + r], wr, wi);
+        // FP64 16-point phases: one element at a time (its two table loads
+        // per element, all hoisted, spill)
+        if constexpr (LONG_S && E >= 16) __builtin_amdgcn_sched_barrier(0);
       });
     });
   };
@@ -789,9 +832,9 @@ __global__ void __launch_bounds__(LINES* SH::TPL)
         sfor<R0>([&](auto r_) PACG_INL {
           constexpr int r = decltype(r_)::value;
           const int s = pos<SH, 0>(tl, c, r);
-          const uint32_t off = line_base + (uint32_t)s * stride;
-          hr_[c * R0 + r] = valid ? hr[off] : TS(0);
-          hi_[c * R0 + r] = valid ? hi[off] : TS(0);
+          const uint32_t off = ld_base + (uint32_t)s * stride;
+          hr_[c * R0 + r] = hr[off];
+          hi_[c * R0 + r] = hi[off];
         });
       });
     }

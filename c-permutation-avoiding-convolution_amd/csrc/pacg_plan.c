@@ -218,10 +218,10 @@ static int default_schedule(int N, int* radix) {
 }
 
 /* Cut one axis' stages into passes, bottom up, taking the longest run with a
- * compiled kernel for the layout the pass will have (P <= 512 on strided
- * lines; up to 2048 on contiguous lines, the bottom pass of the last axis,
- * when that leaves at least 1024 lines).
- * Single stages always have one. */
+ * compiled kernel for the layout the pass will have (up to 512 points, or up
+ * to 2048 where that leaves at least 1024 lines: on contiguous lines, the
+ * bottom pass of the last axis, and on strided FP64 lines of the runs
+ * compiled for them).  Single stages always have one. */
 static int split_axis(axis_t* ax, int need_f32, int mfma) {
   int q = 0;
   ax->npass = 0;
@@ -246,7 +246,8 @@ static int split_axis(axis_t* ax, int need_f32, int mfma) {
       }
     }
     /* this pass' lines are contiguous; passes longer than 512 points only where they
-     * still leave 1024 or more lines to spread over the CUs */
+     * still leave 1024 or more lines to spread over the CUs (and, strided, only in
+     * FP64: need_f32 finds no FP32 kernel for them) */
     const int contig = q == 0 && ax->inner == 1;
     const long long npts = ax->outer * (long long)ax->N * ax->inner;
     for (int len = ax->t - q; len >= 1 && !best; --len) {
@@ -258,7 +259,7 @@ static int split_axis(axis_t* ax, int need_f32, int mfma) {
       if (len > 12) continue;
       long long P = 1;
       for (int i = q; i < q + len; ++i) P *= ax->radix[i];
-      if (P > (contig ? 2048 : 512) || (P > 512 && npts / P < 1024)) continue;
+      if (P > 2048 || (P > 512 && npts / P < 1024)) continue;
       uint32_t code = run_code(ax->radix, q, q + len - 1);
       int lay = PACG_L_STRIDED;
       if (!find_variant(PACG_F64, PACG_F64, PACG_K_DIF, lay, len, code)) {
@@ -492,6 +493,20 @@ int pacg_plan_create(pacg_plan** out, const pacg_desc* d) {
   if (d->fused_axis >= 0 && d->fused_axis < d->nd && pl->ax[d->fused_axis].t > 0) pl->fused_axis = d->fused_axis;
   for (int a = 0; a < d->nd && pl->fused_axis < 0; ++a)
     if (pl->ax[a].t > 0) pl->fused_axis = a;
+  /* ... unless, not requested, the fused pass there would run strided lines of
+   * more than 512 points: such tiles hold 2-4 lines (32 KiB per FP64 line of
+   * 2048 points), and with data and spectrum the fused form gets only the 2-line
+   * tile (2048^2 FP64: 85 us against 48 us for the fused pass on contiguous
+   * lines).  The product then moves to the last axis with stages. */
+  if (d->fused_axis < 0 && pl->fused_axis >= 0) {
+    const axis_t* fx = &pl->ax[pl->fused_axis];
+    if (fx->npass > 0 && fx->pass[0].P > 512 && fx->inner > 1)
+      for (int a = d->nd - 1; a > pl->fused_axis; --a)
+        if (pl->ax[a].t > 0) {
+          pl->fused_axis = a;
+          break;
+        }
+  }
   /* A^T step order: non-fused axes innermost first, then the fused axis; top pass first */
   pl->nsteps = 0;
   pl->npsteps = 0;

@@ -39,6 +39,8 @@ CASES = [
     ((32, 32, 64), [[8, 4], [4, 8], [8, 8]]),
     ((1, 16, 8), [[], [4, 4], [8]]),
     ((4, 1, 8), [[4], [], [8]]),
+    # FP64: axis 0 in one strided 2048-point pass (1024 lines)
+    ((2048, 1024), [[4] * 5 + [2], [4] * 5]),
 ]
 
 

@@ -29,6 +29,11 @@ CASES = [
     # starts on the narrow tiles that give the most workgroups
     ((65536,), [[2] * 16], 0),
     ((65536,), [[4] * 8], 0),
+    # FP64 strided 2048-point lines (2-, 4- and 8-line tiles, 8 or 16 points per thread, rows
+    # folded in LDS): the fused pass on axis 0 (on request), then A^T / conj(A) passes on axis 0
+    # with the product on axis 1 (the default)
+    ((2048, 1024), [[4, 4, 4, 4, 4, 2], [4, 4, 4, 4, 4]], 0, 0),
+    ((2048, 1024), [[4, 4, 4, 4, 4, 2], [4, 4, 4, 4, 4]], 0),
 ]
 
 
@@ -37,15 +42,17 @@ def rnd(seed, shape):
     return g.uniform(-1, 1, shape) + 1j * g.uniform(-1, 1, shape)
 
 
-@pytest.mark.parametrize("dims,scheds,prec", CASES)
-def test_every_tile_shape_of_every_launch(gpu, oracle, dims, scheds, prec):
+@pytest.mark.parametrize("case", CASES)
+def test_every_tile_shape_of_every_launch(gpu, oracle, case):
+    dims, scheds, prec = case[:3]
+    fused_axis = case[3] if len(case) > 3 else -1
     x = rnd(31, dims)
     k = rnd(32, dims)
     y_ref, _ = oracle.pa_convolution(x, k, scheds)
     dt = np.float32 if prec == 1 else np.float64
     tol_ref = 2e-6 if prec == 1 else 1e-12
     tol_tile = 1e-6 if prec == 1 else 1e-14
-    pl = gpu.Plan(dims, schedules=scheds, precision=prec, spectrum_precision=prec)
+    pl = gpu.Plan(dims, schedules=scheds, precision=prec, spectrum_precision=prec, fused_axis=fused_axis)
     pl.set_kernel(k)
     dx = gpu.DeviceComplex.from_numpy(x, dt)
     dy = pl.alloc()
