@@ -35,10 +35,6 @@ N1 = 256
 SCHED = [4, 8, 8]
 SCHEDS32 = [SCHED] * 3
 SCHEDS64 = [SCHED] * 3
-# pair launches (two axis passes per kernel, 3 launches per convolution), reported beside the
-# headline: the middle axis as 4-8-8 so that its radix-4 bottom stage pairs with axis 0 and its
-# 8-8 top with axis 2
-SCHEDS_PAIRS = [[8, 8, 4], [4, 8, 8], [8, 8, 4]]
 
 
 def log(*a):
@@ -694,8 +690,6 @@ def main():
     ap.add_argument("--spectrum", choices=["f32", "f64"], default="f32",
                     help="storage of the FP64-computed spectrum on the headline FP32 path")
     ap.add_argument("--no-tune", action="store_true", help="skip the tile autotuner")
-    ap.add_argument("--pairs", action="store_true",
-                    help="--profile: the pair-launch plan instead of one axis per launch")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched uneven-length workload")
     ap.add_argument("--no-mfma", action="store_true", help="skip the 4096x256 MFMA workload")
     ap.add_argument("--no-extra", action="store_true",
@@ -777,8 +771,7 @@ def main():
     s_bytes = 8 if a.spectrum == "f32" else 16
 
     # ---- FP32 path (headline) -------------------------------------------------
-    pl32 = pacg.Plan([N1] * 3, schedules=SCHEDS_PAIRS if (a.profile and a.pairs) else scheds,
-                     precision=pacg.PREC_F32, spectrum_precision=sprec, pairs=a.profile and a.pairs)
+    pl32 = pacg.Plan([N1] * 3, schedules=scheds, precision=pacg.PREC_F32, spectrum_precision=sprec)
     pl32.set_kernel(k)
     dx32 = pacg.DeviceComplex.from_numpy(x, np.float32)
     dy32 = pl32.alloc()
@@ -807,15 +800,6 @@ def main():
     dxi = pacg.DeviceComplex.from_numpy(x, np.float32)
     ms32_ip = time_convs(pacg, pl32, dxi, dxi, a.steps, a.warmup)
     del dxi
-
-    # ---- FP32 with pair launches (two axis passes per kernel) ---------------------------
-    pl1 = pacg.Plan([N1] * 3, schedules=SCHEDS_PAIRS, precision=pacg.PREC_F32, spectrum_precision=sprec, pairs=True)
-    pl1.set_kernel(k)
-    ms_pairs = time_convs(pacg, pl1, dx32, dy32, a.steps, a.warmup)
-    nl_pairs = pl1.launches_per_convolve()
-    prof_pairs = profile_launches(pacg, pl1, dx32, dy32, max(5, a.steps // 4))
-    y_pairs = dy32.numpy()
-    del pl1
 
     # ---- FP32 path with the other spectrum storage --------------------------------
     alt = "f64" if a.spectrum == "f32" else "f32"
@@ -848,17 +832,14 @@ def main():
     max32 = float(np.abs(y32 - yref).max())
     err64 = float(np.linalg.norm(y64 - yref) / nref)
     err_alt = float(np.linalg.norm(ya - yref) / nref)
-    err_pairs = float(np.linalg.norm(y_pairs - yref) / nref)
     log("rel-L2 fp32 %.3e  fp64 %.3e" % (err32, err64))
 
     # ---- roofline ------------------------------------------------------------
     dom = max(prof32, key=lambda p: p[1])
     dom_bytes = alg_bytes_launch(dom[0], n, 8, s_bytes)
     achieved = dom_bytes / (dom[1] * 1e-3) / 1e9
-    # round trips of the array actually made (2d-1 = 5 one axis per launch, 3 with pair launches)
-    # plus one spectrum read
+    # round trips of the array actually made (2d-1 = 5, one axis per launch) plus one spectrum read
     b_alg = conv_alg_bytes(nl32, n, 8, s_bytes)
-    b_pairs = conv_alg_bytes(nl_pairs, n, 8, s_bytes)
     b_alg64 = conv_alg_bytes(nl64, n, 16, 16)
     conv_gbs = b_alg / (ms32 * 1e-3) / 1e9
     flops = 2 * 5 * n * np.log2(n)  # two transforms (forward x, inverse), 5 N log2 N each
@@ -891,10 +872,6 @@ def main():
         "fp32_in_place": {"ms_per_conv": ms32_ip, "conv_per_s": 1000.0 / ms32_ip,
                           "note": "y overwrites x each step (reference semantics); array + spectrum fit the "
                                   "Infinity Cache"},
-        "fp32_pair_launches": {"schedules": SCHEDS_PAIRS, "ms_per_conv": ms_pairs, "conv_per_s": 1000.0 / ms_pairs,
-                               "launches": nl_pairs, "alg_bytes": b_pairs,
-                               "frac": b_pairs / (ms_pairs * 1e-3) / 1e9 / HBM_PEAK_GBS, "rel_l2": err_pairs,
-                               "kernels": [{"kernel": nm, "ms": t} for nm, t in prof_pairs]},
         "gflops": flops / (ms32 * 1e-3) / 1e9,
         "fp64_plain": {"ms_per_conv": ms64, "conv_per_s": 1000.0 / ms64, "launches": nl64, "alg_bytes": b_alg64,
                        "frac": b_alg64 / (ms64 * 1e-3) / 1e9 / HBM_PEAK_GBS, "rel_l2": err64},

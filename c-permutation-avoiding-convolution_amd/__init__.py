@@ -39,8 +39,7 @@ class PacgError(RuntimeError):
 class _Desc(C.Structure):
     _fields_ = [("nd", C.c_int), ("dims", C.c_int * 3), ("nstages", C.c_int * 3),
                 ("radix", (C.c_int * MAX_STAGES) * 3), ("precision", C.c_int),
-                ("spectrum_precision", C.c_int), ("fused_axis", C.c_int), ("mfma", C.c_int),
-                ("axis_pairs", C.c_int)]
+                ("spectrum_precision", C.c_int), ("fused_axis", C.c_int), ("mfma", C.c_int)]
 
 
 def _load():
@@ -231,12 +230,10 @@ class Plan:
 
     ``schedules``: per axis, a list of radices (stage 0 = unit stride) or None
     for the default; ``radix``: pure radix for every axis (alternative).
-    ``pairs``: 3D plans run passes of two axes as one launch where a pair
-    kernel for the schedules is compiled (default False: one axis per launch).
     """
 
     def __init__(self, dims, schedules=None, radix=None, precision=PREC_F64, spectrum_precision=PREC_F64,
-                 fused_axis=-1, mfma=False, pairs=False):
+                 fused_axis=-1, mfma=False):
         dims = [int(d) for d in dims]
         d = _Desc()
         if radix is not None:
@@ -257,7 +254,6 @@ class Plan:
         d.spectrum_precision = int(spectrum_precision)
         d.fused_axis = int(fused_axis)
         d.mfma = 1 if mfma else 0
-        d.axis_pairs = 1 if pairs else 0
         self.dims = dims
         self.precision = int(precision)
         self.dtype = np.float64 if precision == PREC_F64 else np.float32

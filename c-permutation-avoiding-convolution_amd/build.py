@@ -77,7 +77,7 @@ def build(jobs: int | None = None, incremental: bool = False, verbose: bool = Tr
     finally:
         sys.path.pop(0)
     gen_variants.main(GEN)
-    hdrs = [os.path.join(CSRC, h) for h in ("pacg_pass.hpp", "pacg_pair.hpp", "pacg_internal.h", "pacg_host.h",
+    hdrs = [os.path.join(CSRC, h) for h in ("pacg_pass.hpp", "pacg_internal.h", "pacg_host.h",
                                             "pacg_batch.h")]
     hdrs += [os.path.join(INCLUDE, h) for h in ("pacg.h", "pacg_fft.h")]
     tasks = []

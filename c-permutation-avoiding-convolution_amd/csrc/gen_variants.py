@@ -175,92 +175,9 @@ def emit(prec: str, outdir: str):
     return names
 
 
-# Pair kernels: (pass X shape, pass Y shape, U columns, Y contiguous rows,
-# points per thread, precisions).  3D arrays with a middle axis of 256:
-#   [axis 1 top stages + all of axis 2] with the axis-1 top of 32 points
-#   (E = 8, both precisions) or 64 points (E = 16, FP32), and
-#   [all of axis 0 + axis 1 bottom stage, U columns of axis 2] with a radix-8
-#   bottom and 4 columns (E = 8) or a radix-4 bottom and 8 (E = 8) or 16
-#   (E = 16, FP32) columns; outer and inner axes of 256 or 64.
-PAIRS = []
-for _x in ([8, 4], [4, 8]):
-    for _y in ([8, 8, 4], [4, 8, 8], [8, 8]):
-        PAIRS.append((_x, _y, 1, True, 8, ("double", "float")))
-for _y in ([8, 8, 4], [4, 8, 8]):
-    PAIRS.append(([8, 8], _y, 1, True, 16, ("float",)))
-for _x in ([8, 8, 4], [4, 8, 8], [8, 8]):
-    PAIRS.append((_x, [8], 4, False, 8, ("double", "float")))
-for _x in ([8, 8, 4], [4, 8, 8]):
-    PAIRS.append((_x, [4], 8, False, 8, ("double", "float")))
-    PAIRS.append((_x, [4], 16, False, 16, ("float",)))
-PAIR_UNITS = 6
-
-
-def shape_decl(shape, lge):
-    sc, npph, pc = codes(shape)
-    return "Shape<%d, 0x%Xu, %d, 0x%Xu, %d>" % (len(shape), sc, npph, pc, lge)
-
-
-def emit_pairs(outdir: str):
-    rows = []
-    for prec in ("double", "float"):
-        tag = "f64" if prec == "double" else "f32"
-        pconst = "PACG_F64" if prec == "double" else "PACG_F32"
-        spec_types = ["double"] if prec == "double" else ["float", "double"]
-        for sx, sy, U, yc, E, precs in PAIRS:
-            if prec not in precs:
-                continue
-            PX, EX = geometry(sx, prec)[:2]
-            PY, EY = geometry(sy, prec)[:2]
-            assert E >= EX and E >= EY and PX >= E
-            lge = E.bit_length() - 1
-            # Y keeps its own points per thread when shorter than E (E / PY lines per thread)
-            lgy = lge if PY >= E else 0
-            threads = PX * PY * U // E
-            assert threads <= 1024
-            scx, _, _ = codes(sx)
-            scy, _, _ = codes(sy)
-            for kind in KINDS:
-                # fused kernels hold data and spectrum together: with 16 values per
-                # thread, or FP64 data on 8192-point tiles, they would spill
-                if kind == "PACG_K_FUSED" and (E == 16 or (prec == "double" and PX * PY * U >= 8192)):
-                    continue
-                specs = spec_types if kind == "PACG_K_FUSED" else [prec]
-                for ts in specs:
-                    # ... and with an FP64 spectrum or FP64 data there is no room for the next tile
-                    pf = not (kind == "PACG_K_FUSED" and (prec == "double" or ts == "double"))
-                    fn = "launch_pair<%s, %s, %s, %s, %d, %s, %s, %s>" % (
-                        prec, ts, shape_decl(sx, lge), shape_decl(sy, lgy), U, "true" if yc else "false", kind,
-                        "true" if pf else "false")
-                    sp = "PACG_F64" if ts == "double" else "PACG_F32"
-                    nm = "%s_pair_%s_x%s_y%s_u%d%s_e%d%s" % (
-                        tag, kind[7:].lower(), "x".join(map(str, sx)), "x".join(map(str, sy)), U,
-                        "c" if yc else "", E, "_h64" if (kind == "PACG_K_FUSED" and ts == "double") else "")
-                    rows.append('  {%s, %s, %s, %d, 0x%Xu, %d, 0x%Xu, %d, %d, %d, %d, %d, 0, &%s, "%s"},'
-                                % (pconst, sp, kind, len(sx), scx, len(sy), scy, U, 1 if yc else 0, PX, PY, threads,
-                                   fn, nm))
-    names = []
-    for k in range(PAIR_UNITS):
-        part = rows[k::PAIR_UNITS]
-        nm = "pacg_pairs_%d" % k
-        names.append((nm, len(part)))
-        lines = [
-            "// generated by gen_variants.py -- pair kernel instantiations (part %d)" % k,
-            '#include "../pacg_pair.hpp"',
-            "using namespace pacg;",
-            'extern "C" {',
-            "extern pacg_pair_variant_t %s[];" % nm,
-            "pacg_pair_variant_t %s[] = {" % nm,
-        ] + part + ["};", "}"]
-        with open(os.path.join(outdir, "%s.hip" % nm), "w") as f:
-            f.write("\n".join(lines) + "\n")
-    return names
-
-
 def main(outdir: str):
     os.makedirs(outdir, exist_ok=True)
     tables = emit("double", outdir) + emit("float", outdir)
-    pairs = emit_pairs(outdir)
     idx = ["/* generated by gen_variants.py -- registry of all pass kernels */",
            '#include "../pacg_internal.h"']
     for nm, _ in tables:
@@ -268,15 +185,9 @@ def main(outdir: str):
     idx.append("const pacg_variant_t* const pacg_variant_tables[] = {%s};" % ", ".join(nm for nm, _ in tables))
     idx.append("const int pacg_variant_counts[] = {%s};" % ", ".join(str(c) for _, c in tables))
     idx.append("const int pacg_num_variant_tables = %d;" % len(tables))
-    for nm, _ in pairs:
-        idx.append("extern pacg_pair_variant_t %s[];" % nm)
-    idx.append("const pacg_pair_variant_t* const pacg_pair_tables[] = {%s};" % ", ".join(nm for nm, _ in pairs))
-    idx.append("const int pacg_pair_counts[] = {%s};" % ", ".join(str(c) for _, c in pairs))
-    idx.append("const int pacg_num_pair_tables = %d;" % len(pairs))
     with open(os.path.join(outdir, "pacg_variants_index.c"), "w") as f:
         f.write("\n".join(idx) + "\n")
-    print("generated %d pass variants in %d units, %d pair variants in %d units"
-          % (sum(c for _, c in tables), len(tables), sum(c for _, c in pairs), len(pairs)))
+    print("generated %d pass variants in %d units" % (sum(c for _, c in tables), len(tables)))
 
 
 if __name__ == "__main__":

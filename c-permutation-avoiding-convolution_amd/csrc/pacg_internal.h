@@ -72,62 +72,6 @@ typedef struct {
   int engine;            /* 0: VALU butterflies, 1: radix-8 stages on MFMA (split FP16) */
 } pacg_variant_t;
 
-/* Arguments of one pair launch: pass X (axis a, PX points SX elements apart)
- * and pass Y (axis b > a, PY points SY apart) on tiles of PX x PY x U points,
- * U neighbouring columns at unit stride (U <= SY).  Tile t splits as
- * t = (ox * noy + oy) * nub + ub and holds the points
- *   ox*PX*SX + oy*PY*SY + ub*U + x*SX + y*SY + u. */
-typedef struct {
-  const void* in_re;
-  const void* in_im;
-  void* out_re;
-  void* out_im;
-  const void* spec_re;
-  const void* spec_im;
-  const void* tw_x;      /* w_PX table of pass X */
-  const void* tw_y;      /* w_PY table of pass Y */
-  const void* tw_hi;     /* axis a two-level table (pass X external twiddle) */
-  const void* tw_lo;
-  uint32_t log2_SX;
-  uint32_t log2_SY;
-  uint32_t log2_nub;     /* log2(SY / U) */
-  uint32_t log2_noy;     /* log2(SX / (PY * SY)) */
-  uint32_t n_tiles;
-  uint32_t log2_inner_x; /* within-axis lower index of an X line = j >> log2_inner_x */
-  uint32_t ext_shift_x;
-  uint32_t lo_bits;
-  uint32_t ext_x;        /* 1: pass X carries the external twiddle w_K^{j' k}; then PY * SY
-                            equals the inner size of axis a (one j' per tile) */
-  uint32_t vec16;
-  uint32_t num_cus;      /* compute units of the device (persistent grid size) */
-  double scale;
-  double out_scale;
-} pacg_pair_args_t;
-
-typedef int (*pacg_pair_launch_fn)(const pacg_pair_args_t* a, void* stream);
-
-/* One compiled pair kernel. */
-typedef struct {
-  int prec;
-  int spec_prec;
-  int kind;              /* PACG_K_*: fused needs both passes at the bottom of their axes */
-  int nsx;               /* pass X stages and code */
-  uint32_t codex;
-  int nsy;               /* pass Y stages and code */
-  uint32_t codey;
-  int U;
-  int ycont;             /* 1: Y lines are contiguous rows (SY == 1, U == 1) */
-  int PX, PY, threads;
-  int rank;
-  pacg_pair_launch_fn launch;
-  const char* name;
-} pacg_pair_variant_t;
-
-/* registry of every compiled pair kernel (generated: gen/pacg_variants_index.c) */
-extern const pacg_pair_variant_t* const pacg_pair_tables[];
-extern const int pacg_pair_counts[];
-extern const int pacg_num_pair_tables;
-
 /* registry of every compiled pass kernel (generated: gen/pacg_variants_index.c) */
 extern const pacg_variant_t* const pacg_variant_tables[];
 extern const int pacg_variant_counts[];

@@ -85,15 +85,7 @@ typedef struct {
 
 typedef struct {
   int axis, pass;
-  int pair; /* >= 0: this step is pl->pairs[pair] (two passes in one launch) */
 } step_t;
-
-/* Two passes of two axes run as one launch (pacg_pair.hpp): pass X of axis
- * ax_x and pass Y of axis ax_y > ax_x on tiles of PX x PY x U points. */
-typedef struct {
-  int ax_x, pass_x, ax_y, pass_y, U, ycont;
-  const pacg_pair_variant_t* v[2][4][2]; /* [prec][kind][spec prec] */
-} pair_t;
 
 struct pacg_plan {
   pacg_desc d;
@@ -103,10 +95,6 @@ struct pacg_plan {
   int fused_axis;
   int nsteps;
   step_t steps[3 * MAX_PASS]; /* A^T order */
-  pair_t pairs[2];
-  int npairs;
-  int npsteps;
-  step_t psteps[2]; /* A^T order in the plan precision when pairs are bound */
   void* stream;
   int own_stream;
   void* spec_re;
@@ -162,20 +150,6 @@ static const pacg_variant_t* fill_tile(const pacg_variant_t* v, long long lines)
       if (!same_pass(c, v)) continue;
       const long long bc = (lines + c->lines - 1) / c->lines, bb = (lines + best->lines - 1) / best->lines;
       if (bb >= FILL_BLOCKS ? (bc >= FILL_BLOCKS && c->lines > best->lines) : bc > bb) best = c;
-    }
-  return best;
-}
-
-static const pacg_pair_variant_t* find_pair(int prec, int spec_prec, int kind, int nsx, uint32_t codex, int nsy,
-                                            uint32_t codey, int ycont, int max_u) {
-  const pacg_pair_variant_t* best = NULL;
-  for (int t = 0; t < pacg_num_pair_tables; ++t)
-    for (int i = 0; i < pacg_pair_counts[t]; ++i) {
-      const pacg_pair_variant_t* v = &pacg_pair_tables[t][i];
-      if (v->rank == 0 && v->prec == prec && v->kind == kind && v->nsx == nsx && v->codex == codex &&
-          v->nsy == nsy && v->codey == codey && v->ycont == ycont && v->U <= max_u &&
-          (kind != PACG_K_FUSED || v->spec_prec == spec_prec) && (!best || v->U > best->U))
-        best = v;
     }
   return best;
 }
@@ -320,58 +294,6 @@ static int bind_variants(pacg_plan* pl) {
   return PACG_OK;
 }
 
-/* Binds one pair's kernels in the plan precision (A^T, A and conj(A); the
- * fused form too when fsprec >= 0).  Returns 1 when all exist. */
-static int bind_pair(pair_t* pr, int prec, int nsx, uint32_t cx, int nsy, uint32_t cy, int max_u, int fsprec) {
-  for (int k = 0; k < 4; ++k)
-    for (int sp = 0; sp < 2; ++sp) pr->v[prec][k][sp] = find_pair(prec, sp, k, nsx, cx, nsy, cy, pr->ycont, max_u);
-  if (!pr->v[prec][PACG_K_DIF][prec] || !pr->v[prec][PACG_K_DITC][prec] || !pr->v[prec][PACG_K_DIT][prec]) return 0;
-  return fsprec < 0 || pr->v[prec][PACG_K_FUSED][fsprec] != NULL;
-}
-
-/* 3D plans: run [the top stages of axis 1 + all of axis 2] and [all of axis 0
- * + the bottom stages of axis 1] as one launch each when pair kernels for the
- * schedules are compiled.  Axis 1 is cut into a bottom and a top pass for
- * this; the convolution then takes 3 round trips of the array instead of 5. */
-static int try_pairs(pacg_plan* pl) {
-  const pacg_desc* d = &pl->d;
-  if (d->axis_pairs != 1 || pl->nd != 3 || d->mfma || pl->batch != 1 || d->fused_axis > 0) return 0;
-  axis_t *a0 = &pl->ax[0], *a1 = &pl->ax[1], *a2 = &pl->ax[2];
-  if (a0->t < 1 || a1->t < 2 || a2->t < 1 || a0->npass != 1 || a2->npass != 1) return 0;
-  int fprec = pl->precs[PACG_F32] ? PACG_F32 : PACG_F64;
-  int fsprec = (fprec == PACG_F32 && d->spectrum_precision == PACG_PREC_F32) ? PACG_F32 : PACG_F64;
-  if ((unsigned long long)pl->n * esize(fprec) > (1ULL << 32)) return 0; /* 32-bit byte offsets in tiles */
-  for (int b = 1; b < a1->t; ++b) {
-    uint32_t cbot = run_code(a1->radix, 0, b - 1), ctop = run_code(a1->radix, b, a1->t - 1);
-    pair_t A, B;
-    memset(&A, 0, sizeof A);
-    memset(&B, 0, sizeof B);
-    A.ax_x = 1, A.pass_x = 1, A.ax_y = 2, A.pass_y = 0, A.ycont = 1;
-    B.ax_x = 0, B.pass_x = 0, B.ax_y = 1, B.pass_y = 0, B.ycont = 0;
-    /* pair A: X = axis 1 top (external twiddle: Y = all of axis 2 = axis 1's inner size) */
-    if (!bind_pair(&A, fprec, a1->t - b, ctop, a2->t, a2->pass[0].code, 1, -1)) continue;
-    /* pair B: X = all of axis 0, Y = axis 1 bottom, up to N2 columns */
-    if (!bind_pair(&B, fprec, a0->t, a0->pass[0].code, b, cbot, a2->N, fsprec)) continue;
-    a1->npass = 2;
-    for (int p = 0; p < 2; ++p) {
-      pass_t* ps = &a1->pass[p];
-      memset(ps, 0, sizeof *ps);
-      ps->q_lo = p ? b : 0;
-      ps->q_hi = p ? a1->t - 1 : b - 1;
-      ps->P = 1;
-      for (int i = ps->q_lo; i <= ps->q_hi; ++i) ps->P *= a1->radix[i];
-      ps->L = 1;
-      for (int i = 0; i < ps->q_lo; ++i) ps->L *= a1->radix[i];
-      ps->code = run_code(a1->radix, ps->q_lo, ps->q_hi);
-    }
-    pl->pairs[0] = A;
-    pl->pairs[1] = B;
-    pl->npairs = 2;
-    return 1;
-  }
-  return 0;
-}
-
 static int build_tables(pacg_plan* pl) {
   for (int pr = 0; pr < 2; ++pr) {
     if (!pl->precs[pr]) continue;
@@ -509,21 +431,13 @@ int pacg_plan_create(pacg_plan** out, const pacg_desc* d) {
   }
   /* A^T step order: non-fused axes innermost first, then the fused axis; top pass first */
   pl->nsteps = 0;
-  pl->npsteps = 0;
-  if (try_pairs(pl)) {
-    /* plan precision: two pair launches; FP64 set-up work (the spectrum of an
-     * FP32 plan) keeps the one-axis-per-launch steps below */
-    pl->fused_axis = 0;
-    pl->psteps[pl->npsteps++] = (step_t){1, 1, 0};
-    pl->psteps[pl->npsteps++] = (step_t){0, 0, 1};
-  }
   for (int a = d->nd - 1; a >= 0; --a) {
     if (a == pl->fused_axis) continue;
-    for (int p = pl->ax[a].npass - 1; p >= 0; --p) pl->steps[pl->nsteps++] = (step_t){a, p, -1};
+    for (int p = pl->ax[a].npass - 1; p >= 0; --p) pl->steps[pl->nsteps++] = (step_t){a, p};
   }
   if (pl->fused_axis >= 0)
     for (int p = pl->ax[pl->fused_axis].npass - 1; p >= 0; --p)
-      pl->steps[pl->nsteps++] = (step_t){pl->fused_axis, p, -1};
+      pl->steps[pl->nsteps++] = (step_t){pl->fused_axis, p};
   if ((rc = bind_variants(pl)) != PACG_OK) goto fail;
   if (pacg_stream_create(&pl->stream)) {
     rc = pacg_fail(PACG_ERR_HIP, "stream creation failed");
@@ -619,17 +533,6 @@ int pacg_plan_describe(const pacg_plan* pl, char* buf, int buflen) {
   const step_t* S = steps_for(pl, pidx(pl), &ns);
   buf[0] = 0;
   for (int s = 0; s < ns && off < buflen; ++s) {
-    if (S[s].pair >= 0) {
-      const pair_t* pr = &pl->pairs[S[s].pair];
-      const pass_t* px = &pl->ax[pr->ax_x].pass[pr->pass_x];
-      const pass_t* py = &pl->ax[pr->ax_y].pass[pr->pass_y];
-      int pp = pl->d.precision == PACG_PREC_F32;
-      const pacg_pair_variant_t* v = pr->v[pp][PACG_K_DIF][pp];
-      off += snprintf(buf + off, (size_t)(buflen - off), "%saxis%d[q%d..%d P=%d L=%d]+axis%d[q%d..%d P=%d L=%d %s]",
-                      s ? " " : "", pr->ax_x, px->q_lo, px->q_hi, px->P, px->L, pr->ax_y, py->q_lo, py->q_hi, py->P,
-                      py->L, v ? v->name : "?");
-      continue;
-    }
     const axis_t* ax = &pl->ax[S[s].axis];
     const pass_t* ps = &ax->pass[S[s].pass];
     const pacg_variant_t* v = ps->v[pl->d.precision == PACG_PREC_F32][PACG_K_DIF][pl->d.precision == PACG_PREC_F32];
@@ -640,50 +543,8 @@ int pacg_plan_describe(const pacg_plan* pl, char* buf, int buflen) {
 }
 
 /* ---- launching ------------------------------------------------------------ */
-static int launch_pair_step(pacg_plan* pl, int pi, int prec, int kind, int sprec, const void* ir, const void* ii,
-                            void* orr, void* oi, const void* sr, const void* si, double scale, double out_scale) {
-  const pair_t* pr = &pl->pairs[pi];
-  const pacg_pair_variant_t* v = pr->v[prec][kind][sprec];
-  if (!v) return pacg_fail(PACG_ERR_NOKERNEL, "no pair kernel for kind %d (pair %d)", kind, pi);
-  const axis_t* ax = &pl->ax[pr->ax_x];
-  const axis_t* ay = &pl->ax[pr->ax_y];
-  const pass_t* px = &ax->pass[pr->pass_x];
-  const pass_t* py = &ay->pass[pr->pass_y];
-  long long SX = (long long)px->L * ax->inner, SY = (long long)py->L * ay->inner;
-  pacg_pair_args_t A;
-  memset(&A, 0, sizeof A);
-  A.in_re = ir;
-  A.in_im = ii;
-  A.out_re = orr;
-  A.out_im = oi;
-  A.spec_re = sr;
-  A.spec_im = si;
-  A.tw_x = px->twp[prec];
-  A.tw_y = py->twp[prec];
-  A.tw_hi = ax->tw_hi[prec];
-  A.tw_lo = ax->tw_lo[prec];
-  A.log2_SX = (uint32_t)ilog2(SX);
-  A.log2_SY = (uint32_t)ilog2(SY);
-  A.log2_nub = (uint32_t)ilog2(SY / v->U);
-  A.log2_noy = (uint32_t)ilog2(SX / ((long long)v->PY * SY));
-  A.n_tiles = (uint32_t)(pl->n / ((long long)v->PX * v->PY * v->U));
-  A.log2_inner_x = (uint32_t)ilog2(ax->inner);
-  A.ext_x = px->L > 1;
-  A.ext_shift_x = (uint32_t)ilog2(ax->N / ((long long)px->P * px->L));
-  A.lo_bits = (uint32_t)ax->lo_bits;
-  A.scale = scale;
-  A.out_scale = out_scale;
-  A.vec16 = (((uintptr_t)ir | (uintptr_t)ii | (uintptr_t)orr | (uintptr_t)oi) & 15u) == 0;
-  A.num_cus = (uint32_t)pacg_num_cus();
-  int rc = v->launch(&A, pl->stream);
-  if (rc) return pacg_fail(PACG_ERR_HIP, "launch of %s failed: %s", v->name, pacg_hip_error_string(rc));
-  return PACG_OK;
-}
-
 static int launch_step(pacg_plan* pl, step_t st, int prec, int kind, int sprec, const void* ir, const void* ii,
                        void* orr, void* oi, const void* sr, const void* si, double scale, double out_scale) {
-  if (st.pair >= 0)
-    return launch_pair_step(pl, st.pair, prec, kind, sprec, ir, ii, orr, oi, sr, si, scale, out_scale);
   axis_t* ax = &pl->ax[st.axis];
   pass_t* ps = &ax->pass[st.pass];
   const pacg_variant_t* v = ps->v[prec][kind][sprec];
@@ -717,12 +578,9 @@ static int launch_step(pacg_plan* pl, step_t st, int prec, int kind, int sprec, 
 
 static int pidx(const pacg_plan* pl) { return pl->d.precision == PACG_PREC_F32 ? PACG_F32 : PACG_F64; }
 
-/* launch steps (A^T order) for work in precision prec */
+/* launch steps (A^T order) */
 static const step_t* steps_for(const pacg_plan* pl, int prec, int* n) {
-  if (pl->npsteps > 0 && prec == pidx(pl)) {
-    *n = pl->npsteps;
-    return pl->psteps;
-  }
+  (void)prec;
   *n = pl->nsteps;
   return pl->steps;
 }
@@ -806,7 +664,7 @@ static int run_conv(pacg_plan* pl, int prec, const void* xr, const void* xi, voi
  * buffers, out of place x -> y, and keep the fastest.  Returns the number of
  * launches whose kernel changed, or minus an error code. */
 /* The kernel slot of launch i of a convolution and its current kernel (NULL
- * for a pair launch or an index out of range).  Any compiled kernel of the
+ * for an index out of range).  Any compiled kernel of the
  * same pass (same_pass: precision, kind, layout, stages, spectrum precision
  * and engine; any lines per workgroup) can take the slot. */
 static const pacg_variant_t* launch_slot(pacg_plan* pl, int i, const pacg_variant_t*** slot) {
@@ -814,7 +672,6 @@ static const pacg_variant_t* launch_slot(pacg_plan* pl, int i, const pacg_varian
   int kind;
   if (i < 0 || i >= conv_launches(pl)) return NULL;
   conv_launch(pl, i, &st, &kind);
-  if (st.pair >= 0) return NULL;
   int prec = pidx(pl), vsp = kind == PACG_K_FUSED ? pl->d.spectrum_precision : prec;
   *slot = &pl->ax[st.axis].pass[st.pass].v[prec][kind][vsp];
   return **slot;
@@ -877,7 +734,6 @@ int pacg_plan_autotune(pacg_plan* pl, const void* xr, const void* xi, void* yr, 
     step_t st;
     int kind;
     conv_launch(pl, i, &st, &kind);
-    if (st.pair >= 0) continue; /* one compiled tile per pair */
     pass_t* ps = &pl->ax[st.axis].pass[st.pass];
     int vsp = kind == PACG_K_FUSED ? sp : prec;
     const pacg_variant_t* cur = ps->v[prec][kind][vsp];
@@ -973,11 +829,6 @@ int pacg_profile_convolve(pacg_plan* pl, const void* xr, const void* xi, void* y
       int kind;
       conv_launch(pl, i, &st, &kind);
       int vsp = kind == PACG_K_FUSED ? sp : prec;
-      if (st.pair >= 0) {
-        const pacg_pair_variant_t* pv = pl->pairs[st.pair].v[prec][kind][vsp];
-        names[i] = pv ? pv->name : "?";
-        continue;
-      }
       const pacg_variant_t* v = pl->ax[st.axis].pass[st.pass].v[prec][kind][vsp];
       names[i] = v ? v->name : "?";
     }

@@ -70,13 +70,12 @@ typedef struct {
   int radix[3][PACG_MAX_STAGES];          /* entries in {2, 4, 8} */
   int precision;                          /* PACG_PREC_F64 or PACG_PREC_F32: data arrays */
   int spectrum_precision;                 /* storage of A^T k (always computed in FP64) */
-  int fused_axis;                         /* axis whose last pass carries the product; -1 = 0 */
+  int fused_axis;                         /* axis whose last pass carries the product; -1 = the
+                                             first axis with stages, or the last one when the
+                                             first one's would run strided lines of more than
+                                             512 points */
   int mfma;                               /* FP32 plans: run radix-8 stages as F_8 (x) I contractions
                                              on the matrix cores (split FP16, FP32 accumulation) */
-  int axis_pairs;                         /* 3D: 1 = run passes of two axes as one launch where a
-                                             pair kernel is compiled (3 round trips per convolution
-                                             instead of 5); 0 = one axis per launch (default: faster
-                                             on MI355X at 256^3, the pair tiles are latency-bound) */
 } pacg_desc;
 
 typedef struct pacg_plan pacg_plan;
@@ -137,8 +136,8 @@ int pacg_plan_autotune(pacg_plan* plan, const void* x_re, const void* x_im, void
 
 /* Tile shapes of launch i (0-based, in launch order) of pacg_convolve: writes
  * up to max_names kernel names (static strings) of the compiled kernels that
- * can run it and returns how many there are, or minus an error code (a pair
- * launch has one kernel and no choice).  pacg_plan_pin_launch makes the named
+ * can run it and returns how many there are, or minus an error code.
+ * pacg_plan_pin_launch makes the named
  * one current, as the autotuner would; it drops a captured graph. */
 int pacg_plan_launch_candidates(pacg_plan* plan, int launch, const char** names, int max_names);
 int pacg_plan_pin_launch(pacg_plan* plan, int launch, const char* name);
