@@ -496,6 +496,37 @@ def _timed_calls(pacg, stream, reset, call, reps):
     return float(np.mean(ms)), float(np.std(ms))
 
 
+def bench_schedule_search(pacg, synth, steps, warmup):
+    """Radix-schedule search (pacg_plan_search) from each configuration's own schedules: the schedules it
+    keeps and both plans' rates (tiles autotuned, HIP events, device-resident, FP64 spectrum for FP64)."""
+    out = {}
+    cases = (("3d_256cubed_f32", "3d_256cubed", SCHEDS32, pacg.PREC_F32, pacg.PREC_F32),
+             ("3d_256cubed_f64", "3d_256cubed", SCHEDS64, pacg.PREC_F64, pacg.PREC_F64),
+             ("2d_2048x2048_f64", "2d_2048x2048", [[4, 4, 4, 4, 4, 2]] * 2, pacg.PREC_F64, pacg.PREC_F64),
+             ("1d_65536_f64", "1d_65536", [[2] * 16], pacg.PREC_F64, pacg.PREC_F64))
+    for name, wl, start, prec, sp in cases:
+        x, k = synth.workload_inputs(wl)
+        t0 = time.time()
+        found, changed = pacg.search_schedules(x.shape, schedules=start, precision=prec, spectrum_precision=sp,
+                                               reps=5)
+        search_s = time.time() - t0
+        dt = np.float32 if prec == pacg.PREC_F32 else np.float64
+        dx = pacg.DeviceComplex.from_numpy(x, dt)
+        res = {}
+        for tag, sc in (("start", start), ("found", found)):
+            pl = pacg.Plan(x.shape, schedules=sc, precision=prec, spectrum_precision=sp)
+            pl.set_kernel(k)
+            dy = pl.alloc()
+            pl.autotune(dx, dy, 3)
+            res[tag] = {"schedules": sc, "ms_per_conv": time_convs(pacg, pl, dx, dy, steps, warmup),
+                        "plan": pl.describe()}
+            del pl, dy
+        out[name] = {"axes_changed": changed, "search_s": search_s, **res,
+                     "speedup": res["start"]["ms_per_conv"] / res["found"]["ms_per_conv"]}
+        del dx
+    return out
+
+
 def _errs(y, ref):
     return {"rel_l2": float(np.linalg.norm(y - ref) / np.linalg.norm(ref)), "max_abs": float(np.abs(y - ref).max())}
 
@@ -902,6 +933,8 @@ def main():
         out["fft_based_vs_pa_1d_2e24"] = bench_fft_based_1d(pacg, a.steps)
         log("per-element ufft / uifft / permutation ...")
         out["per_element"] = bench_per_element(pacg)
+        log("radix-schedule search ...")
+        out["schedule_search"] = bench_schedule_search(pacg, synth, a.steps, a.warmup)
     d_us, g_us = pacg.launch_floor(2000)
     out["launch_floor"] = {"empty_kernel_direct_us": d_us, "empty_graph_replay_us": g_us,
                            "note": "back-to-back launches of an empty kernel, and replays of a HIP graph holding "

@@ -71,6 +71,8 @@ def _load():
     L.pacg_profile_convolve.argtypes = [_vp, _vp, _vp, _vp, _vp, C.c_int, C.POINTER(C.c_float),
                                         C.POINTER(C.c_char_p), C.c_int]
     L.pacg_device_sync.argtypes = []
+    L.pacg_plan_search.argtypes = [C.POINTER(_Desc), C.c_int]
+    L.pacg_schedule_candidates.argtypes = [C.c_int, _ip, C.c_int, C.c_int]
     L.pacg_batch_create.argtypes = [C.POINTER(_vp), C.c_int, _ip]
     L.pacg_batch_destroy.argtypes = [_vp]
     L.pacg_batch_total.argtypes = [_vp]
@@ -225,6 +227,56 @@ class DeviceComplex:
 # ---------------------------------------------------------------------------
 # extended plan API
 # ---------------------------------------------------------------------------
+def _make_desc(dims, schedules, radix, precision, spectrum_precision, fused_axis, mfma) -> _Desc:
+    """pacg_desc for the Plan arguments (schedules per axis, or one radix for all, or defaults)."""
+    d = _Desc()
+    if radix is not None:
+        arr = (C.c_int * 3)(*(list(dims) + [0] * (3 - len(dims))))
+        _check(lib.pacg_desc_init(C.byref(d), len(dims), arr, int(radix), int(precision)), "pacg_desc_init")
+    else:
+        d.nd = len(dims)
+        for a, n in enumerate(dims):
+            d.dims[a] = n
+            sc = (schedules or [None] * len(dims))[a]
+            if isinstance(sc, str) and sc == "batch":
+                d.nstages[a] = -1  # batch axis: rows convolved independently
+            elif sc:
+                d.nstages[a] = len(sc)
+                for q, r in enumerate(sc):
+                    d.radix[a][q] = int(r)
+        d.precision = int(precision)
+    d.spectrum_precision = int(spectrum_precision)
+    d.fused_axis = int(fused_axis)
+    d.mfma = 1 if mfma else 0
+    return d
+
+
+def search_schedules(dims, schedules=None, precision=PREC_F64, spectrum_precision=PREC_F64, fused_axis=-1,
+                     mfma=False, reps: int = 5):
+    """pacg_plan_search: the fastest radix schedule per axis (coordinate descent from `schedules`,
+    defaults where None).  Returns (schedules, number of axes that changed)."""
+    dims = [int(x) for x in dims]
+    d = _make_desc(dims, schedules, None, precision, spectrum_precision, fused_axis, mfma)
+    rc = lib.pacg_plan_search(C.byref(d), int(reps))
+    if rc < 0:
+        _check(-rc, "pacg_plan_search")
+    out = []
+    for a in range(len(dims)):
+        ns = d.nstages[a]
+        out.append("batch" if ns < 0 else ([int(d.radix[a][q]) for q in range(ns)] if ns > 0 else None))
+    return out, int(rc)
+
+
+def schedule_candidates(n: int):
+    """pacg_schedule_candidates: the schedules pacg_plan_search tries for a length n."""
+    cnt = lib.pacg_schedule_candidates(int(n), None, 0, MAX_STAGES)
+    if cnt < 0:
+        _check(-cnt, "pacg_schedule_candidates")
+    buf = (C.c_int * (max(cnt, 1) * MAX_STAGES))()
+    lib.pacg_schedule_candidates(int(n), buf, cnt, MAX_STAGES)
+    return [[v for v in buf[i * MAX_STAGES:(i + 1) * MAX_STAGES] if v] for i in range(cnt)]
+
+
 class Plan:
     """Device plan for one shape / schedule / precision (pacg_plan_create).
 
@@ -235,25 +287,7 @@ class Plan:
     def __init__(self, dims, schedules=None, radix=None, precision=PREC_F64, spectrum_precision=PREC_F64,
                  fused_axis=-1, mfma=False):
         dims = [int(d) for d in dims]
-        d = _Desc()
-        if radix is not None:
-            arr = (C.c_int * 3)(*(dims + [0] * (3 - len(dims))))
-            _check(lib.pacg_desc_init(C.byref(d), len(dims), arr, int(radix), int(precision)), "pacg_desc_init")
-        else:
-            d.nd = len(dims)
-            for a, n in enumerate(dims):
-                d.dims[a] = n
-                sc = (schedules or [None] * len(dims))[a]
-                if isinstance(sc, str) and sc == "batch":
-                    d.nstages[a] = -1  # batch axis: rows convolved independently
-                elif sc:
-                    d.nstages[a] = len(sc)
-                    for q, r in enumerate(sc):
-                        d.radix[a][q] = int(r)
-            d.precision = int(precision)
-        d.spectrum_precision = int(spectrum_precision)
-        d.fused_axis = int(fused_axis)
-        d.mfma = 1 if mfma else 0
+        d = _make_desc(dims, schedules, radix, precision, spectrum_precision, fused_axis, mfma)
         self.dims = dims
         self.precision = int(precision)
         self.dtype = np.float64 if precision == PREC_F64 else np.float32

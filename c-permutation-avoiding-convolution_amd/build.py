@@ -50,7 +50,7 @@ HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall",
              "-I", CSRC, "-I", INCLUDE]
 C_FLAGS = ["-O2", "-std=c11", "-fPIC", "-Wall", "-Wextra", "-Wno-unused-parameter",
            "-I", CSRC, "-I", INCLUDE]
-HOST_C = ["pacg_plan.c", "pacg_host.c", "pacg_dropin.c", "pacg_batch.c"]
+HOST_C = ["pacg_plan.c", "pacg_host.c", "pacg_dropin.c", "pacg_batch.c", "pacg_search.c"]
 HIP_SRC = ["pacg_aux.hip", "pacg_batch.hip", "pacg_mfma.hip"]
 
 

@@ -134,6 +134,22 @@ int pacg_graph_replay(pacg_plan* plan);
  * error code.  Drops a captured graph (capture again afterwards). */
 int pacg_plan_autotune(pacg_plan* plan, const void* x_re, const void* x_im, void* y_re, void* y_im, int reps);
 
+/* Radix-schedule search: for the shape, precisions and switches of d, tries
+ * the candidate schedules of each transformed axis in turn with the other axes
+ * fixed (two sweeps), each as a plan with its tiles autotuned, timing `reps`
+ * convolutions of seeded random data on scratch buffers it allocates, and
+ * writes the fastest schedules into d->nstages / d->radix (a schedule replaces
+ * the current one only when it is more than 1 % faster).  Candidates of a
+ * length 2^t: every mix of radix-8, -4 and at most one radix-2 stage, each with
+ * its radices descending and ascending from stage 0 (pacg_schedule_candidates).
+ * The caller's schedules (defaults where nstages is 0) are the starting point
+ * and must make a plan.  Returns how many axes changed schedule, or -error. */
+int pacg_plan_search(pacg_desc* d, int reps);
+/* The candidate schedules of a length n = 2^t, one per row of `out`
+ * (max_stages ints, stage 0 first, the unused tail 0): writes up to max_cand
+ * rows and returns how many candidates there are, or -error. */
+int pacg_schedule_candidates(int n, int* out, int max_cand, int max_stages);
+
 /* Tile shapes of launch i (0-based, in launch order) of pacg_convolve: writes
  * up to max_names kernel names (static strings) of the compiled kernels that
  * can run it and returns how many there are, or minus an error code.
