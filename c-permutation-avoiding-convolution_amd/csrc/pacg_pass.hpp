@@ -448,14 +448,15 @@ __device__ __forceinline__ constexpr int row_fold(int s) {
 // ---------------------------------------------------------------------------
 // the pass kernel
 // ---------------------------------------------------------------------------
-// Minimum waves per SIMD the register allocation must allow.  Strided FP64
-// tiles of long lines hold 2-8 lines per workgroup of 256-1024 threads, and LDS
-// leaves room for 4 waves per SIMD (two 512-thread or four 256-thread
-// workgroups per CU): without the bound the allocator takes up to 254 VGPRs
-// for the smaller workgroups (2 waves per SIMD).  Other tiles: no bound.
+// Minimum waves per SIMD the register allocation must allow.  FP64 tiles of
+// lines of 2048+ points: LDS leaves room for 4 waves per SIMD (strided: two
+// 512-thread or four 256-thread workgroups per CU; contiguous: four one-line
+// workgroups), and without the bound the allocator takes up to 254 VGPRs for
+// the smaller strided workgroups (2 waves per SIMD) and 130 for the fused
+// contiguous one of 8 points per thread (3 waves).  Other tiles: no bound.
 template <typename T, class SH, int LAYOUT>
 constexpr int pass_min_waves() {
-  return (sizeof(T) == 8 && LAYOUT == PACG_L_STRIDED && SH::P >= 2048) ? 4 : 1;
+  return (sizeof(T) == 8 && SH::P >= 2048) ? 4 : 1;
 }
 
 template <typename T, typename TS, class SH, int KIND, int LAYOUT, int LINES>
