@@ -339,3 +339,24 @@ def test_dropin_calls_from_several_threads(gpu):
         t.join(timeout=120)
     assert not any(t.is_alive() for t in threads)
     assert not errors, errors
+
+
+def test_strided_2048_point_pass_with_external_twiddle(gpu, oracle):
+    """A strided 2048-point FP64 pass that is not the bottom pass of its axis (65536 points = 8 * 4 *
+    2048, lower span 32) multiplies by the pass-external twiddle w_K^{j' k}: convolution and the three
+    operators against the oracle."""
+    dims = (65536, 32)
+    scheds = [[8] + [4] * 6 + [2], [4, 8]]
+    pl = gpu.Plan(dims, schedules=scheds)
+    assert "P=2048 L=32 f64_dif_s_4x4x4x4x4x2" in pl.describe(), pl.describe()
+    x = rnd(51, dims)
+    k = rnd(52, dims)
+    y_ref, _ = oracle.pa_convolution(x, k, scheds)
+    pl.set_kernel(k)
+    dx = gpu.DeviceComplex.from_numpy(x)
+    dy = pl.alloc()
+    pl.convolve(dx, dy)
+    assert rel_l2(dy.numpy(), y_ref) < TOL64
+    for op, kind in ((gpu.OP_TRANS, TRANS), (gpu.OP_CONJ, CONJ), (gpu.OP_FWD, A)):
+        pl.execute(op, dx, dy)
+        assert rel_l2(dy.numpy(), oracle.butterflies_nd(x, scheds, kind)) < TOL64, op
