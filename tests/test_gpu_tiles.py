@@ -34,6 +34,9 @@ CASES = [
     # with the product on axis 1 (the default)
     ((2048, 1024), [[4, 4, 4, 4, 4, 2], [4, 4, 4, 4, 4]], 0, 0),
     ((2048, 1024), [[4, 4, 4, 4, 4, 2], [4, 4, 4, 4, 4]], 0),
+    # ... on an axis whose inner size (2) is smaller than a tile's line count: one tile spans
+    # several outer indices
+    ((512, 2048, 2), [[8, 8, 8], [4, 4, 4, 4, 4, 2], [2]], 0),
 ]
 
 
