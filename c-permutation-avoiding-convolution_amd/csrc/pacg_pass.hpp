@@ -459,7 +459,10 @@ constexpr int pass_min_waves() {
   return (sizeof(T) == 8 && SH::P >= 2048) ? 4 : 1;
 }
 
-template <typename T, typename TS, class SH, int KIND, int LAYOUT, int LINES>
+// SNT: the fused pass streams the spectrum with non-temporal loads (chosen at
+// launch from the plan's flag; a separate instantiation, since a branch on the
+// flag inside one kernel costs registers in both forms)
+template <typename T, typename TS, class SH, int KIND, int LAYOUT, int LINES, bool SNT = false>
 __global__ void __launch_bounds__(LINES* SH::TPL) __attribute__((amdgpu_waves_per_eu(pass_min_waves<T, SH, LAYOUT>())))
     pass_kernel(const pacg_pass_args_t a) {
   static_assert(SH::valid(), "invalid pass shape");
@@ -834,8 +837,13 @@ __global__ void __launch_bounds__(LINES* SH::TPL) __attribute__((amdgpu_waves_pe
           constexpr int r = decltype(r_)::value;
           const int s = pos<SH, 0>(tl, c, r);
           const uint32_t off = ld_base + (uint32_t)s * stride;
-          hr_[c * R0 + r] = hr[off];
-          hi_[c * R0 + r] = hi[off];
+          if constexpr (SNT) {
+            hr_[c * R0 + r] = __builtin_nontemporal_load(hr + off);
+            hi_[c * R0 + r] = __builtin_nontemporal_load(hi + off);
+          } else {
+            hr_[c * R0 + r] = hr[off];
+            hi_[c * R0 + r] = hi[off];
+          }
         });
       });
     }
@@ -893,6 +901,15 @@ int launch_pass(const pacg_pass_args_t* a, void* stream) {
     blocks = (a->n_o + LINES - 1) / LINES;
   }
   if (blocks == 0) return 0;
+  // FP64-stored spectra stream past the caches when the plan asks for it
+  // (the array and the spectrum exceed the Infinity Cache together)
+  if constexpr (KIND == PACG_K_FUSED && sizeof(TS) == 8) {
+    if (a->spec_nt) {
+      hipLaunchKernelGGL((pass_kernel<T, TS, SH, KIND, LAYOUT, LINES, true>), dim3(blocks), dim3(LINES * SH::TPL), 0,
+                         (hipStream_t)stream, *a);
+      return (int)hipGetLastError();
+    }
+  }
   hipLaunchKernelGGL((pass_kernel<T, TS, SH, KIND, LAYOUT, LINES>), dim3(blocks), dim3(LINES * SH::TPL), 0,
                      (hipStream_t)stream, *a);
   return (int)hipGetLastError();
