@@ -902,8 +902,9 @@ int launch_pass(const pacg_pass_args_t* a, void* stream) {
   }
   if (blocks == 0) return 0;
   // FP64-stored spectra stream past the caches when the plan asks for it
-  // (the array and the spectrum exceed the Infinity Cache together)
-  if constexpr (KIND == PACG_K_FUSED && sizeof(TS) == 8) {
+  // (strided tiles of an array and a spectrum that exceed the Infinity Cache
+  // together)
+  if constexpr (KIND == PACG_K_FUSED && sizeof(TS) == 8 && LAYOUT == PACG_L_STRIDED) {
     if (a->spec_nt) {
       hipLaunchKernelGGL((pass_kernel<T, TS, SH, KIND, LAYOUT, LINES, true>), dim3(blocks), dim3(LINES * SH::TPL), 0,
                          (hipStream_t)stream, *a);

@@ -571,16 +571,18 @@ static int launch_step(pacg_plan* pl, step_t st, int prec, int kind, int sprec, 
   A.scale = scale;
   A.out_scale = out_scale;
   A.vec16 = (((uintptr_t)ir | (uintptr_t)ii | (uintptr_t)orr | (uintptr_t)oi) & 15u) == 0;
-  if (kind == PACG_K_FUSED) {
+  if (kind == PACG_K_FUSED && sprec == PACG_F64 && v->layout == PACG_L_STRIDED) {
     /* When the array and the spectrum together exceed the 256 MiB Infinity
      * Cache, an FP64-stored spectrum (read once per convolution) is streamed
      * past it, so that the array stays cached for the passes either side:
      * 256^3 FP32 with an FP64 spectrum (384 MiB) 0.290 -> 0.262 ms, 256^3
-     * FP64 (512 MiB) 0.539 -> 0.481 ms.  A working set that fits (2048^2
-     * FP64, 128 MiB) keeps the default loads, which are faster there; so do
-     * FP32 spectra (non-temporal loads slowed the 256^3 FP32 fused pass). */
-    const unsigned long long bytes = (unsigned long long)pl->n * 2ull *
-                                     ((prec == PACG_F32 ? 4ull : 8ull) + (sprec == PACG_F32 ? 4ull : 8ull));
+     * FP64 (512 MiB) 0.539 -> 0.481 ms.  Only on strided tiles, whose loads
+     * read whole runs of lines per instruction: a contiguous tile reads each
+     * element's points R apart, and without cache allocation every element
+     * fetches the lines again (1D 2^24 FP64 fused pass 151 -> 216 us).  A
+     * working set that fits (2048^2 FP64, 128 MiB) keeps the default loads,
+     * as do FP32 spectra (non-temporal loads slowed the 256^3 FP32 pass). */
+    const unsigned long long bytes = (unsigned long long)pl->n * 2ull * ((prec == PACG_F32 ? 4ull : 8ull) + 8ull);
     A.spec_nt = bytes > (256ull << 20);
   }
   int rc = v->launch(&A, pl->stream);
