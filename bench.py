@@ -731,6 +731,9 @@ def main():
     ap.add_argument("--profile-mfma", action="store_true",
                     help="only run the 4096x256 convolution `steps` times with the MFMA plan, then with the VALU "
                          "plan (for rocprofv3 counters)")
+    ap.add_argument("--profile-fp64-3d", action="store_true",
+                    help="only run the 256^3 plans whose fused pass streams an FP64 spectrum (FP64 plain, then FP32 "
+                         "with the spectrum stored in FP64), autotuned, `steps` times each (for rocprofv3)")
     ap.add_argument("--profile-batched", action="store_true",
                     help="only run the batched workload (graph replay) `steps` times (for rocprofv3)")
     ap.add_argument("--profile", action="store_true",
@@ -772,6 +775,24 @@ def main():
             pl.convolve(dx, dy)
         pl.sync()
         print(json.dumps({"profile_fp64_2d": True, "steps": a.steps, "plan": pl.describe()}))
+        return
+
+    if a.profile_fp64_3d:
+        x3, k3 = synth.workload_inputs("3d_256cubed")
+        plans = {}
+        for name, prec in (("fp64", pacg.PREC_F64), ("fp32_spectrum_f64", pacg.PREC_F32)):
+            pl = pacg.Plan(x3.shape, schedules=SCHEDS64, precision=prec, spectrum_precision=pacg.PREC_F64)
+            pl.set_kernel(k3)
+            dx = pacg.DeviceComplex.from_numpy(x3, np.float64 if prec == pacg.PREC_F64 else np.float32)
+            dy = pl.alloc()
+            pl.autotune(dx, dy, 3)  # the tiles the bench line reports
+            pl.sync()
+            for _ in range(a.steps):
+                pl.convolve(dx, dy)
+            pl.sync()
+            plans[name] = pl.describe()
+            del pl, dx, dy
+        print(json.dumps({"profile_fp64_3d": True, "steps": a.steps, "plans": plans}))
         return
 
     if a.profile_mfma:
