@@ -94,6 +94,7 @@ def _load():
     L.pacg_plan_autotune.argtypes = [_vp, _vp, _vp, _vp, _vp, C.c_int]
     L.pacg_plan_launch_candidates.argtypes = [_vp, C.c_int, C.POINTER(C.c_char_p), C.c_int]
     L.pacg_plan_pin_launch.argtypes = [_vp, C.c_int, C.c_char_p]
+    L.pacg_plan_stream_input.argtypes = [_vp, C.c_int]
     L.pacg_bw_probe.argtypes = [C.c_int, _vp, _vp, C.c_uint64, _vp]
     L.pacg_hbm_probe.argtypes = [C.c_int, _vp, _vp, C.c_uint64, C.c_int, _vp, C.POINTER(C.c_float)]
     L.pacg_launch_floor.argtypes = [C.c_int, _vp, C.POINTER(C.c_float), C.POINTER(C.c_float)]
@@ -373,6 +374,11 @@ class Plan:
         """Run launch `launch` of convolve with the named kernel (one of candidates(launch))."""
         if lib.pacg_plan_pin_launch(self.ptr, int(launch), name.encode()):
             raise PacgError("pacg_plan_pin_launch failed: %s" % last_error())
+
+    def stream_input(self, on: bool = True):
+        """Load x non-temporally in the first launch of convolve (the autotuner's other choice)."""
+        if lib.pacg_plan_stream_input(self.ptr, int(bool(on))):
+            raise PacgError("pacg_plan_stream_input failed: %s" % last_error())
 
     def profile(self, x: DeviceComplex, y: DeviceComplex, reps: int = 10):
         """Mean duration (ms) of every kernel launch of one convolve, HIP events on the plan stream."""

@@ -49,6 +49,7 @@ typedef struct {
   uint32_t ext;          /* 1: apply the pass-external twiddle w_K^{j' k} */
   uint32_t vec16;        /* 1: in/out base pointers are 16-byte aligned (vector access allowed) */
   uint32_t spec_nt;      /* fused, strided, FP64 spectrum: load it non-temporally (array + spectrum exceed the Infinity Cache) */
+  uint32_t in_nt;        /* A^T pass reading the caller's x (first launch of a convolution): load it non-temporally */
   double scale;          /* fused: factor of the spectral product (1/n), applied at the store */
   double out_scale;      /* factor applied at the final store (1 or 1/n) */
 } pacg_pass_args_t;

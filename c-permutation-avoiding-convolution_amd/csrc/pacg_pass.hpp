@@ -405,6 +405,12 @@ struct Vec4<float> {
     o[3] = v.w;
   }
   static __device__ __forceinline__ float4 join(const float* o) { return make_float4(o[0], o[1], o[2], o[3]); }
+  // a 16-byte non-temporal load (the builtin takes native vector types)
+  static __device__ __forceinline__ float4 load_nt(const float* p) {
+    typedef float nv __attribute__((ext_vector_type(4)));
+    const nv t = __builtin_nontemporal_load(reinterpret_cast<const nv*>(p));
+    return make_float4(t[0], t[1], t[2], t[3]);
+  }
 };
 template <>
 struct Vec4<double> {
@@ -414,6 +420,11 @@ struct Vec4<double> {
     o[1] = v.y;
   }
   static __device__ __forceinline__ double2 join(const double* o) { return make_double2(o[0], o[1]); }
+  static __device__ __forceinline__ double2 load_nt(const double* p) {
+    typedef double nv __attribute__((ext_vector_type(2)));
+    const nv t = __builtin_nontemporal_load(reinterpret_cast<const nv*>(p));
+    return make_double2(t[0], t[1]);
+  }
 };
 
 // Row bits of a strided FP32 tile with 16 lines (one bit) or 8 lines (two
@@ -459,10 +470,13 @@ constexpr int pass_min_waves() {
   return (sizeof(T) == 8 && SH::P >= 2048) ? 4 : 1;
 }
 
-// SNT: the fused pass streams the spectrum with non-temporal loads (chosen at
-// launch from the plan's flag; a separate instantiation, since a branch on the
-// flag inside one kernel costs registers in both forms)
-template <typename T, typename TS, class SH, int KIND, int LAYOUT, int LINES, bool SNT = false>
+// NTM, the load policy (chosen at launch from the plan's flags; a separate
+// instantiation, since a branch on a flag inside one kernel costs registers in
+// both forms): bit 0, the fused pass streams the spectrum with non-temporal
+// loads; bit 1, an A^T pass loads its input non-temporally (the caller's x,
+// which one convolution reads once).
+enum { NT_SPEC = 1, NT_IN = 2 };
+template <typename T, typename TS, class SH, int KIND, int LAYOUT, int LINES, int NTM = 0>
 __global__ void __launch_bounds__(LINES* SH::TPL) __attribute__((amdgpu_waves_per_eu(pass_min_waves<T, SH, LAYOUT>())))
     pass_kernel(const pacg_pass_args_t a) {
   static_assert(SH::valid(), "invalid pass shape");
@@ -644,8 +658,14 @@ __global__ void __launch_bounds__(LINES* SH::TPL) __attribute__((amdgpu_waves_pe
           const uint32_t off = ld_base + (uint32_t)pos<SH, 0>(tl, c, 0);
           sfor<R / VW>([&](auto v_) PACG_INL {
             constexpr int v = decltype(v_)::value;
-            const V4 xr4 = reinterpret_cast<const V4*>(br + off)[v];
-            const V4 xi4 = reinterpret_cast<const V4*>(bi + off)[v];
+            V4 xr4, xi4;
+            if constexpr ((NTM & NT_IN) != 0) {
+              xr4 = Vec4<T>::load_nt(br + off + v * VW);
+              xi4 = Vec4<T>::load_nt(bi + off + v * VW);
+            } else {
+              xr4 = reinterpret_cast<const V4*>(br + off)[v];
+              xi4 = reinterpret_cast<const V4*>(bi + off)[v];
+            }
             Vec4<T>::split(xr4, &vr[c * R + v * VW]);
             Vec4<T>::split(xi4, &vi[c * R + v * VW]);
           });
@@ -659,8 +679,13 @@ __global__ void __launch_bounds__(LINES* SH::TPL) __attribute__((amdgpu_waves_pe
         constexpr int r = decltype(r_)::value;
         const int s = pos<SH, PH>(tl, c, r);
         const uint32_t off = ld_base + (uint32_t)s * stride;
-        vr[c * R + r] = br[off];
-        vi[c * R + r] = bi[off];
+        if constexpr ((NTM & NT_IN) != 0) {
+          vr[c * R + r] = __builtin_nontemporal_load(br + off);
+          vi[c * R + r] = __builtin_nontemporal_load(bi + off);
+        } else {
+          vr[c * R + r] = br[off];
+          vi[c * R + r] = bi[off];
+        }
       });
     });
   };
@@ -837,7 +862,7 @@ __global__ void __launch_bounds__(LINES* SH::TPL) __attribute__((amdgpu_waves_pe
           constexpr int r = decltype(r_)::value;
           const int s = pos<SH, 0>(tl, c, r);
           const uint32_t off = ld_base + (uint32_t)s * stride;
-          if constexpr (SNT) {
+          if constexpr ((NTM & NT_SPEC) != 0) {
             hr_[c * R0 + r] = __builtin_nontemporal_load(hr + off);
             hi_[c * R0 + r] = __builtin_nontemporal_load(hi + off);
           } else {
@@ -903,10 +928,18 @@ int launch_pass(const pacg_pass_args_t* a, void* stream) {
   if (blocks == 0) return 0;
   // FP64-stored spectra stream past the caches when the plan asks for it
   // (strided tiles of an array and a spectrum that exceed the Infinity Cache
-  // together)
+  // together); so does the input of a convolution's first A^T pass when the
+  // plan's autotuner found that faster
   if constexpr (KIND == PACG_K_FUSED && sizeof(TS) == 8 && LAYOUT == PACG_L_STRIDED) {
     if (a->spec_nt) {
-      hipLaunchKernelGGL((pass_kernel<T, TS, SH, KIND, LAYOUT, LINES, true>), dim3(blocks), dim3(LINES * SH::TPL), 0,
+      hipLaunchKernelGGL((pass_kernel<T, TS, SH, KIND, LAYOUT, LINES, NT_SPEC>), dim3(blocks), dim3(LINES * SH::TPL), 0,
+                         (hipStream_t)stream, *a);
+      return (int)hipGetLastError();
+    }
+  }
+  if constexpr (KIND == PACG_K_DIF) {
+    if (a->in_nt) {
+      hipLaunchKernelGGL((pass_kernel<T, TS, SH, KIND, LAYOUT, LINES, NT_IN>), dim3(blocks), dim3(LINES * SH::TPL), 0,
                          (hipStream_t)stream, *a);
       return (int)hipGetLastError();
     }

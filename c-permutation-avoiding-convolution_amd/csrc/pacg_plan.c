@@ -103,6 +103,7 @@ struct pacg_plan {
   void* scr_re;
   void* scr_im;
   void* graph;
+  int stream_x; /* the first A^T launch of a convolution loads x non-temporally (set by the autotuner) */
   int precs[2]; /* which precisions have tables */
   long long batch; /* length of a batch axis 0 (1 when none) */
 };
@@ -539,12 +540,13 @@ int pacg_plan_describe(const pacg_plan* pl, char* buf, int buflen) {
     off += snprintf(buf + off, (size_t)(buflen - off), "%saxis%d[q%d..%d P=%d L=%d %s]", s ? " " : "", S[s].axis,
                     ps->q_lo, ps->q_hi, ps->P, ps->L, v ? v->name : "?");
   }
+  if (pl->stream_x && off < buflen) snprintf(buf + off, (size_t)(buflen - off), " x:nt");
   return ns > 0 ? 2 * ns - 1 : 1;
 }
 
 /* ---- launching ------------------------------------------------------------ */
 static int launch_step(pacg_plan* pl, step_t st, int prec, int kind, int sprec, const void* ir, const void* ii,
-                       void* orr, void* oi, const void* sr, const void* si, double scale, double out_scale) {
+                       void* orr, void* oi, const void* sr, const void* si, double scale, double out_scale, int in_nt) {
   axis_t* ax = &pl->ax[st.axis];
   pass_t* ps = &ax->pass[st.pass];
   const pacg_variant_t* v = ps->v[prec][kind][sprec];
@@ -585,6 +587,7 @@ static int launch_step(pacg_plan* pl, step_t st, int prec, int kind, int sprec, 
     const unsigned long long bytes = (unsigned long long)pl->n * 2ull * ((prec == PACG_F32 ? 4ull : 8ull) + 8ull);
     A.spec_nt = bytes > (256ull << 20);
   }
+  A.in_nt = kind == PACG_K_DIF && in_nt;
   int rc = v->launch(&A, pl->stream);
   if (rc) return pacg_fail(PACG_ERR_HIP, "launch of %s failed: %s", v->name, pacg_hip_error_string(rc));
   return PACG_OK;
@@ -612,7 +615,7 @@ static int run_all(pacg_plan* pl, int prec, int kind, const void* ir, const void
     step_t st = (kind == PACG_K_DIF) ? S[i] : S[ns - 1 - i];
     const void* a = i == 0 ? ir : orr;
     const void* b = i == 0 ? ii : oi;
-    int rc = launch_step(pl, st, prec, kind, prec, a, b, orr, oi, NULL, NULL, 1.0, i == ns - 1 ? out_scale : 1.0);
+    int rc = launch_step(pl, st, prec, kind, prec, a, b, orr, oi, NULL, NULL, 1.0, i == ns - 1 ? out_scale : 1.0, 0);
     if (rc) return rc;
   }
   return PACG_OK;
@@ -661,7 +664,7 @@ static int run_conv_ev(pacg_plan* pl, int prec, const void* xr, const void* xi, 
     if (events) HIPCHK(pacg_event_record(events[i], pl->stream), "event record");
     int rc = launch_step(pl, st, prec, kind, kind == PACG_K_FUSED ? sprec : prec, i == 0 ? xr : yr,
                          i == 0 ? xi : yi, yr, yi, kind == PACG_K_FUSED ? sr : NULL, kind == PACG_K_FUSED ? si : NULL,
-                         kind == PACG_K_FUSED ? scale : 1.0, 1.0);
+                         kind == PACG_K_FUSED ? scale : 1.0, 1.0, i == 0 && pl->stream_x);
     if (rc) return rc;
   }
   if (events) HIPCHK(pacg_event_record(events[nl], pl->stream), "event record");
@@ -736,6 +739,16 @@ int pacg_plan_pin_launch(pacg_plan* pl, int launch, const char* name) {
   return pacg_fail(PACG_ERR_ARG, "'%s' is not a tile shape of launch %d (current: %s)", name, launch, cur->name);
 }
 
+int pacg_plan_stream_input(pacg_plan* pl, int on) {
+  if (!pl) return pacg_fail(PACG_ERR_ARG, "pacg_plan_stream_input: NULL plan");
+  pl->stream_x = on != 0;
+  if (pl->graph) {
+    pacg_graph_destroy(pl->graph);
+    pl->graph = NULL;
+  }
+  return PACG_OK;
+}
+
 int pacg_plan_autotune(pacg_plan* pl, const void* xr, const void* xi, void* yr, void* yi, int reps) {
   if (!pl || !xr || !xi || !yr || !yi || reps < 1) return -pacg_fail(PACG_ERR_ARG, "pacg_plan_autotune: bad argument");
   if (!pl->has_spec) return -pacg_fail(PACG_ERR_STATE, "no kernel spectrum set");
@@ -776,10 +789,10 @@ int pacg_plan_autotune(pacg_plan* pl, const void* xr, const void* xi, void* yr, 
         const void* s_re = kind == PACG_K_FUSED ? pl->spec_re : NULL;
         const void* s_im = kind == PACG_K_FUSED ? pl->spec_im : NULL;
         double scale = kind == PACG_K_FUSED ? (double)pl->batch / (double)pl->n : 1.0;
-        if ((rc = launch_step(pl, st, prec, kind, vsp, xr, xi, yr, yi, s_re, s_im, scale, 1.0))) break;
+        if ((rc = launch_step(pl, st, prec, kind, vsp, xr, xi, yr, yi, s_re, s_im, scale, 1.0, 0))) break;
         if (pacg_event_record(e0, pl->stream)) rc = pacg_fail(PACG_ERR_HIP, "event record failed");
         for (int r = 0; r < reps && !rc; ++r)
-          rc = launch_step(pl, st, prec, kind, vsp, xr, xi, yr, yi, s_re, s_im, scale, 1.0);
+          rc = launch_step(pl, st, prec, kind, vsp, xr, xi, yr, yi, s_re, s_im, scale, 1.0, 0);
         if (rc) break;
         if (pacg_event_record(e1, pl->stream) || pacg_stream_sync(pl->stream)) {
           rc = pacg_fail(PACG_ERR_HIP, "autotune timing failed");
@@ -801,6 +814,40 @@ int pacg_plan_autotune(pacg_plan* pl, const void* xr, const void* xi, void* yr, 
     }
     ps->v[prec][kind][vsp] = best;
     if (best != cur) ++changed;
+  }
+  /* Load policy of the first launch, an A^T pass that reads the caller's x
+   * (read once per convolution): default or non-temporal loads, by timing
+   * whole convolutions both ways (two rounds each, each keeps its faster
+   * round); the policy the plan had stays unless the other is faster by more
+   * than 2 %.  A streamed x leaves the Infinity Cache to y and the spectrum,
+   * which the later launches read: 256^3 FP64 0.490 -> 0.451 ms; where they fit
+   * beside x anyway (256^3 FP32) the default loads stay. */
+  step_t st0;
+  int k0 = -1;
+  if (!rc && nl >= 2) conv_launch(pl, 0, &st0, &k0);
+  if (k0 == PACG_K_DIF) {
+    float tb[2] = {1e30f, 1e30f};
+    const int had = pl->stream_x;
+    for (int round = 0; round < 2 && !rc; ++round)
+      for (int f = 0; f < 2 && !rc; ++f) {
+        pl->stream_x = f;
+        if ((rc = run_conv(pl, prec, xr, xi, yr, yi, pl->spec_re, pl->spec_im, sp))) break;
+        if (pacg_event_record(e0, pl->stream)) {
+          rc = pacg_fail(PACG_ERR_HIP, "event record failed");
+          break;
+        }
+        for (int r = 0; r < reps && !rc; ++r) rc = run_conv(pl, prec, xr, xi, yr, yi, pl->spec_re, pl->spec_im, sp);
+        if (rc) break;
+        if (pacg_event_record(e1, pl->stream) || pacg_stream_sync(pl->stream)) {
+          rc = pacg_fail(PACG_ERR_HIP, "autotune timing failed");
+          break;
+        }
+        float ms = 0.0f;
+        pacg_event_elapsed_ms(&ms, e0, e1);
+        if (ms < tb[f]) tb[f] = ms;
+      }
+    pl->stream_x = rc ? had : had ? !(tb[0] < 0.98f * tb[1]) : tb[1] < 0.98f * tb[0];
+    if (pl->stream_x != had) ++changed;
   }
   pacg_event_destroy(e0);
   pacg_event_destroy(e1);

@@ -130,8 +130,12 @@ int pacg_graph_replay(pacg_plan* plan);
 
 /* Autotuner: times every compiled tile shape of each launch of pacg_convolve
  * on these device buffers (x -> y, `reps` launches each) and keeps the
- * fastest per launch.  Returns how many launches changed kernel, or minus an
- * error code.  Drops a captured graph (capture again afterwards). */
+ * fastest per launch; then times whole convolutions with the first launch (an
+ * A^T pass reading x) loading x by default or non-temporally, and keeps the
+ * faster policy (non-temporal: x, read once, leaves the Infinity Cache to y
+ * and the spectrum; pacg_plan_describe then ends in " x:nt").  Returns how many
+ * launches changed kernel or load policy, or minus an error code.  Drops a
+ * captured graph (capture again afterwards). */
 int pacg_plan_autotune(pacg_plan* plan, const void* x_re, const void* x_im, void* y_re, void* y_im, int reps);
 
 /* Radix-schedule search: for the shape, precisions and switches of d, tries
@@ -157,6 +161,11 @@ int pacg_schedule_candidates(int n, int* out, int max_cand, int max_stages);
  * one current, as the autotuner would; it drops a captured graph. */
 int pacg_plan_launch_candidates(pacg_plan* plan, int launch, const char** names, int max_names);
 int pacg_plan_pin_launch(pacg_plan* plan, int launch, const char* name);
+/* Sets the load policy of the first launch of pacg_convolve, as the autotuner
+ * would: non-temporal loads of x when `on` is nonzero (effective where that
+ * launch is an A^T pass, that is where a convolution has two or more launches).
+ * Drops a captured graph. */
+int pacg_plan_stream_input(pacg_plan* plan, int on);
 
 /* Timing aid: runs `reps` convolutions with a HIP event before every kernel
  * launch (on the plan stream) and writes the mean duration of launch i to

@@ -99,3 +99,50 @@ def test_small_arrays_start_on_tiles_that_fill_the_cus(gpu):
     big = gpu.Plan((65536, 256), schedules=["batch", [2] * 8], precision=gpu.PREC_F64).describe()
     assert "f64_dif_c_2x2x2x2x2x2x2x2_h_l4" in small, small
     assert "f64_dif_c_2x2x2x2x2x2x2x2_h_l8" in big, big
+
+
+STREAM_CASES = [
+    # first launch a contiguous A^T pass (axis 2), FP64 and FP32
+    ((16, 32, 64), [[4, 4], [8, 4], [8, 8]], 0),
+    ((16, 32, 64), [[4, 4], [8, 4], [8, 8]], 1),
+    # a strided A^T pass first (2048-point FP64 lines, the product on axis 1)
+    ((2048, 1024), [[4, 4, 4, 4, 4, 2], [4, 4, 4, 4, 4]], 0),
+    # 1D: the top pass, strided
+    ((65536,), [[2] * 16], 0),
+    # a one-phase contiguous first pass (16-point lines: 16-byte loads)
+    ((64, 16), [[8, 8], [4, 4]], 1),
+]
+
+
+@pytest.mark.parametrize("case", STREAM_CASES)
+def test_streamed_input_matches_default_loads(gpu, oracle, case):
+    """The first launch's non-temporal loads of x (the autotuner's other load policy) are a
+    cache hint: the arithmetic is the same, though the two loads are separate kernel builds whose
+    FP32 contractions (which product fuses into an FMA) may differ, so the two convolutions agree
+    to rounding (as two tile shapes do); the default one matches the oracle.  The autotuner then
+    keeps one of the two policies and the result."""
+    dims, scheds, prec = case
+    x = rnd(41, dims)
+    k = rnd(42, dims)
+    dt = np.float32 if prec == 1 else np.float64
+    pl = gpu.Plan(dims, schedules=scheds, precision=prec, spectrum_precision=prec)
+    pl.set_kernel(k)
+    dx = gpu.DeviceComplex.from_numpy(x, dt)
+    dy = pl.alloc()
+    pl.convolve(dx, dy)
+    y0 = dy.numpy()
+    y_ref, _ = oracle.pa_convolution(x, k, scheds)
+    assert rel_l2(y0, y_ref) < (2e-6 if prec == 1 else 1e-12)
+    assert "x:nt" not in pl.describe()
+    pl.stream_input(True)
+    assert pl.describe().endswith(" x:nt"), pl.describe()
+    dy.upload(np.zeros(dims, complex))
+    pl.convolve(dx, dy)
+    tol_tile = 1e-6 if prec == 1 else 1e-14
+    d = rel_l2(dy.numpy(), y0)
+    assert d < tol_tile, d
+    pl.stream_input(False)
+    assert "x:nt" not in pl.describe()
+    pl.autotune(dx, dy, 2)
+    pl.convolve(dx, dy)
+    assert rel_l2(dy.numpy(), y0) < tol_tile

@@ -58,20 +58,24 @@ def test_2d_2048x2048_dropin_r2_matches_reference_algorithm(gpu, oracle, synth):
     assert rel_l2((hr + 1j * hi).reshape(x.shape), ath_ref) < 1e-12
 
 
-@pytest.mark.parametrize("prec,sprec,tol,scheds,launches", [
-    (0, 0, 1e-12, [[8, 8, 4]] * 3, 5),
-    (1, 1, 2e-6, [[4, 8, 8]] * 3, 5),
-    (1, 1, 2e-6, [[8, 8, 4]] * 3, 5),
-    (1, 1, 2e-6, [[8, 8, 4], [4, 8, 8], [8, 8, 4]], 5),
-    (1, 0, 2e-6, [[4, 8, 8]] * 3, 5),
+@pytest.mark.parametrize("prec,sprec,tol,scheds,launches,stream_x", [
+    (0, 0, 1e-12, [[8, 8, 4]] * 3, 5, False),
+    (0, 0, 1e-12, [[4, 8, 8]] * 3, 5, True),
+    (1, 1, 2e-6, [[4, 8, 8]] * 3, 5, False),
+    (1, 1, 2e-6, [[8, 8, 4]] * 3, 5, False),
+    (1, 1, 2e-6, [[8, 8, 4], [4, 8, 8], [8, 8, 4]], 5, False),
+    (1, 0, 2e-6, [[4, 8, 8]] * 3, 5, False),
 ])
-def test_3d_256cubed(gpu, synth, prec, sprec, tol, scheds, launches):
+def test_3d_256cubed(gpu, synth, prec, sprec, tol, scheds, launches, stream_x):
     """3D 256^3, radix-8 schedules, Gaussian stencil (sum 1): FP64 and FP32 (spectrum stored in
     FP32 or FP64) against numpy, plus the delta-kernel identity at full size (one pass per axis:
     5 launches).  FP64 arrays and FP64-stored spectra exceed the Infinity Cache together (512 and
-    384 MiB), so their fused pass streams the spectrum with non-temporal loads."""
+    384 MiB), so their fused pass streams the spectrum with non-temporal loads; one FP64 case
+    also streams x in the first launch (the load policy the autotuner picks there)."""
     x, k = synth.workload_inputs("3d_256cubed")
     pl = gpu.Plan(x.shape, schedules=scheds, precision=prec, spectrum_precision=sprec)
+    if stream_x:
+        pl.stream_input(True)
     assert pl.launches_per_convolve() == launches, pl.describe()
     pl.set_kernel(k)
     dt = np.float64 if prec == 0 else np.float32
