@@ -817,18 +817,19 @@ int pacg_plan_autotune(pacg_plan* pl, const void* xr, const void* xi, void* yr, 
   }
   /* Load policy of the first launch, an A^T pass that reads the caller's x
    * (read once per convolution): default or non-temporal loads, by timing
-   * whole convolutions both ways (two rounds each, each keeps its faster
-   * round); the policy the plan had stays unless the other is faster by more
-   * than 2 %.  A streamed x leaves the Infinity Cache to y and the spectrum,
-   * which the later launches read: 256^3 FP64 0.490 -> 0.451 ms; where they fit
-   * beside x anyway (256^3 FP32) the default loads stay. */
+   * whole convolutions both ways (at least 8 per timing, three rounds, each
+   * policy keeps its fastest round); the policy the plan had stays unless the
+   * other is faster by more than 1 % (the gain is 2-8 %; with 3 convolutions
+   * per timing and a 2 % margin, noise decided).  A streamed x leaves the
+   * Infinity Cache to y and the spectrum, which the later launches read:
+   * 256^3 FP32 0.235 -> 0.230 ms, 256^3 FP64 0.490 -> 0.451 ms. */
   step_t st0;
   int k0 = -1;
   if (!rc && nl >= 2) conv_launch(pl, 0, &st0, &k0);
   if (k0 == PACG_K_DIF) {
     float tb[2] = {1e30f, 1e30f};
-    const int had = pl->stream_x;
-    for (int round = 0; round < 2 && !rc; ++round)
+    const int had = pl->stream_x, nconv = reps > 8 ? reps : 8;
+    for (int round = 0; round < 3 && !rc; ++round)
       for (int f = 0; f < 2 && !rc; ++f) {
         pl->stream_x = f;
         if ((rc = run_conv(pl, prec, xr, xi, yr, yi, pl->spec_re, pl->spec_im, sp))) break;
@@ -836,7 +837,7 @@ int pacg_plan_autotune(pacg_plan* pl, const void* xr, const void* xi, void* yr, 
           rc = pacg_fail(PACG_ERR_HIP, "event record failed");
           break;
         }
-        for (int r = 0; r < reps && !rc; ++r) rc = run_conv(pl, prec, xr, xi, yr, yi, pl->spec_re, pl->spec_im, sp);
+        for (int r = 0; r < nconv && !rc; ++r) rc = run_conv(pl, prec, xr, xi, yr, yi, pl->spec_re, pl->spec_im, sp);
         if (rc) break;
         if (pacg_event_record(e1, pl->stream) || pacg_stream_sync(pl->stream)) {
           rc = pacg_fail(PACG_ERR_HIP, "autotune timing failed");
@@ -846,7 +847,7 @@ int pacg_plan_autotune(pacg_plan* pl, const void* xr, const void* xi, void* yr, 
         pacg_event_elapsed_ms(&ms, e0, e1);
         if (ms < tb[f]) tb[f] = ms;
       }
-    pl->stream_x = rc ? had : had ? !(tb[0] < 0.98f * tb[1]) : tb[1] < 0.98f * tb[0];
+    pl->stream_x = rc ? had : had ? !(tb[0] < 0.99f * tb[1]) : tb[1] < 0.99f * tb[0];
     if (pl->stream_x != had) ++changed;
   }
   pacg_event_destroy(e0);
