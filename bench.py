@@ -507,18 +507,18 @@ def bench_schedule_search(pacg, synth, steps, warmup):
     for name, wl, start, prec, sp in cases:
         x, k = synth.workload_inputs(wl)
         t0 = time.time()
-        found, changed = pacg.search_schedules(x.shape, schedules=start, precision=prec, spectrum_precision=sp,
-                                               reps=5)
+        found, changed, fused = pacg.search_schedules(x.shape, schedules=start, precision=prec,
+                                                      spectrum_precision=sp, reps=5)
         search_s = time.time() - t0
         dt = np.float32 if prec == pacg.PREC_F32 else np.float64
         dx = pacg.DeviceComplex.from_numpy(x, dt)
         res = {}
-        for tag, sc in (("start", start), ("found", found)):
-            pl = pacg.Plan(x.shape, schedules=sc, precision=prec, spectrum_precision=sp)
+        for tag, sc, fa in (("start", start, -1), ("found", found, fused)):
+            pl = pacg.Plan(x.shape, schedules=sc, precision=prec, spectrum_precision=sp, fused_axis=fa)
             pl.set_kernel(k)
             dy = pl.alloc()
             pl.autotune(dx, dy, 3)
-            res[tag] = {"schedules": sc, "ms_per_conv": time_convs(pacg, pl, dx, dy, steps, warmup),
+            res[tag] = {"schedules": sc, "fused_axis": fa, "ms_per_conv": time_convs(pacg, pl, dx, dy, steps, warmup),
                         "plan": pl.describe()}
             del pl, dy
         out[name] = {"axes_changed": changed, "search_s": search_s, **res,

@@ -255,7 +255,9 @@ def _make_desc(dims, schedules, radix, precision, spectrum_precision, fused_axis
 def search_schedules(dims, schedules=None, precision=PREC_F64, spectrum_precision=PREC_F64, fused_axis=-1,
                      mfma=False, reps: int = 5):
     """pacg_plan_search: the fastest radix schedule per axis (coordinate descent from `schedules`,
-    defaults where None).  Returns (schedules, number of axes that changed)."""
+    defaults where None), then, when `fused_axis` is -1, the fastest axis for the fused pass.
+    Returns (schedules, number of axes whose schedule changed, fused axis: -1 for the default
+    rule)."""
     dims = [int(x) for x in dims]
     d = _make_desc(dims, schedules, None, precision, spectrum_precision, fused_axis, mfma)
     rc = lib.pacg_plan_search(C.byref(d), int(reps))
@@ -265,7 +267,7 @@ def search_schedules(dims, schedules=None, precision=PREC_F64, spectrum_precisio
     for a in range(len(dims)):
         ns = d.nstages[a]
         out.append("batch" if ns < 0 else ([int(d.radix[a][q]) for q in range(ns)] if ns > 0 else None))
-    return out, int(rc)
+    return out, int(rc), int(d.fused_axis)
 
 
 def schedule_candidates(n: int):

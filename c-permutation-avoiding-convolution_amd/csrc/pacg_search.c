@@ -7,7 +7,8 @@
  * axis' stride and the precision.  pacg_plan_search tries, axis by axis, the
  * candidate schedules of that axis' length with the other axes fixed
  * (coordinate descent, two sweeps), each as a plan of its own with its tile
- * autotuner run, and keeps the fastest convolution.
+ * autotuner run, and keeps the fastest convolution; then, unless the caller
+ * chose the axis of the fused pass, it tries each axis for it.
  *
  * Candidates of a length 2^t: every count (n8, n4, n2) of radix-8, -4 and -2
  * stages with 3 n8 + 2 n4 + n2 = t and at most one radix-2 stage, each in two
@@ -173,10 +174,25 @@ int pacg_plan_search(pacg_desc* d, int reps) {
         }
       }
     }
+  /* the axis of the fused pass, unless the caller chose one: each transformed
+   * axis in turn with the schedules found (-1, the plan's default rule, stays
+   * unless an explicit axis is more than 1 % faster) */
+  if (d->fused_axis < 0)
+    for (int a = 0; a < cur.nd; ++a) {
+      if (cur.nstages[a] < 0 || cur.dims[a] == 1) continue;
+      pacg_desc trial = cur;
+      trial.fused_axis = a;
+      const double ms = time_desc(&trial, &b, reps);
+      if (ms > 0 && ms < 0.99 * best) {
+        best = ms;
+        cur = trial;
+      }
+    }
   for (int a = 0; a < cur.nd; ++a) {
     d->nstages[a] = cur.nstages[a];
     memcpy(d->radix[a], cur.radix[a], sizeof cur.radix[a]);
   }
+  d->fused_axis = cur.fused_axis;
   pacg_clear_error();
 out:
   free(host);

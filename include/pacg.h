@@ -143,7 +143,10 @@ int pacg_plan_autotune(pacg_plan* plan, const void* x_re, const void* x_im, void
  * fixed (two sweeps), each as a plan with its tiles autotuned, timing `reps`
  * convolutions of seeded random data on scratch buffers it allocates, and
  * writes the fastest schedules into d->nstages / d->radix (a schedule replaces
- * the current one only when it is more than 1 % faster).  Candidates of a
+ * the current one only when it is more than 1 % faster).  When d->fused_axis
+ * is -1 it then tries each transformed axis for the fused pass and writes the
+ * fastest into d->fused_axis (-1, the default rule, unless an axis is more
+ * than 1 % faster).  Candidates of a
  * length 2^t: every mix of radix-8, -4 and at most one radix-2 stage, each with
  * its radices descending and ascending from stage 0 (pacg_schedule_candidates).
  * The caller's schedules (defaults where nstages is 0) are the starting point

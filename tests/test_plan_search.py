@@ -33,8 +33,9 @@ def test_candidates_reject_bad_lengths(pacg):
 @pytest.mark.gpu
 @pytest.mark.parametrize("dims,prec", [((64, 512), 0), ((16, 64, 256), 1), ((4096,), 0)])
 def test_searched_schedules_keep_parity(gpu, oracle, dims, prec):
-    scheds, changed = gpu.search_schedules(dims, precision=prec, spectrum_precision=prec, reps=3)
+    scheds, changed, fused = gpu.search_schedules(dims, precision=prec, spectrum_precision=prec, reps=3)
     assert 0 <= changed <= len(dims)
+    assert -1 <= fused < len(dims)
     for n, sc in zip(dims, scheds):
         assert sc is None or int(np.prod(sc)) == n, (n, sc)
     scheds = [sc if sc is not None else gpu.schedule_candidates(n)[0] for n, sc in zip(dims, scheds)]
@@ -42,7 +43,7 @@ def test_searched_schedules_keep_parity(gpu, oracle, dims, prec):
     x = g.uniform(-1, 1, dims) + 1j * g.uniform(-1, 1, dims)
     k = g.uniform(-1, 1, dims) + 1j * g.uniform(-1, 1, dims)
     y_ref, _ = oracle.pa_convolution(x, k, scheds)
-    pl = gpu.Plan(dims, schedules=scheds, precision=prec, spectrum_precision=prec)
+    pl = gpu.Plan(dims, schedules=scheds, precision=prec, spectrum_precision=prec, fused_axis=fused)
     pl.set_kernel(k)
     dt = np.float32 if prec else np.float64
     dx = gpu.DeviceComplex.from_numpy(x, dt)
