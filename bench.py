@@ -731,6 +731,9 @@ def main():
     ap.add_argument("--profile-mfma", action="store_true",
                     help="only run the 4096x256 convolution `steps` times with the MFMA plan, then with the VALU "
                          "plan (for rocprofv3 counters)")
+    ap.add_argument("--profile-fp64-1d", action="store_true",
+                    help="only run the 1D 2^16 FP64 plan (radix 2), autotuned, `steps` times back to back, then "
+                         "`steps` empty-kernel launches (for rocprofv3)")
     ap.add_argument("--profile-fp64-3d", action="store_true",
                     help="only run the 256^3 plans whose fused pass streams an FP64 spectrum (FP64 plain, then FP32 "
                          "with the spectrum stored in FP64), autotuned, `steps` times each (for rocprofv3)")
@@ -775,6 +778,24 @@ def main():
             pl.convolve(dx, dy)
         pl.sync()
         print(json.dumps({"profile_fp64_2d": True, "steps": a.steps, "plan": pl.describe()}))
+        return
+
+    if a.profile_fp64_1d:
+        x1, k1 = synth.workload_inputs("1d_65536")
+        pl = pacg.Plan(x1.shape, schedules=[[2] * 16], precision=pacg.PREC_F64)
+        pl.set_kernel(k1)
+        dx = pacg.DeviceComplex.from_numpy(x1)
+        dy = pl.alloc()
+        pl.autotune(dx, dy, 3)
+        pl.sync()
+        t = pacg.Timer(pl.stream)
+        t.start()
+        for _ in range(a.steps):
+            pl.convolve(dx, dy)
+        ms = t.stop() / a.steps
+        floor = pacg.launch_floor(a.steps)
+        print(json.dumps({"profile_fp64_1d": True, "steps": a.steps, "plan": pl.describe(), "ms_per_conv": ms,
+                          "launch_floor": floor}))
         return
 
     if a.profile_fp64_3d:
