@@ -825,8 +825,10 @@ int pacg_plan_autotune(pacg_plan* pl, const void* xr, const void* xi, void* yr, 
    * 256^3 FP32 0.235 -> 0.230 ms, 256^3 FP64 0.490 -> 0.451 ms. */
   step_t st0;
   int k0 = -1;
-  if (!rc && nl >= 2) conv_launch(pl, 0, &st0, &k0);
-  if (k0 == PACG_K_DIF) {
+  const pacg_variant_t** slot0 = NULL;
+  const pacg_variant_t* v0 = (!rc && nl >= 2) ? launch_slot(pl, 0, &slot0) : NULL;
+  if (v0) conv_launch(pl, 0, &st0, &k0);
+  if (k0 == PACG_K_DIF && v0->engine == 0) { /* MFMA pass kernels have one load policy */
     float tb[2] = {1e30f, 1e30f};
     const int had = pl->stream_x, nconv = reps > 8 ? reps : 8;
     for (int round = 0; round < 3 && !rc; ++round)

@@ -166,8 +166,9 @@ int pacg_plan_launch_candidates(pacg_plan* plan, int launch, const char** names,
 int pacg_plan_pin_launch(pacg_plan* plan, int launch, const char* name);
 /* Sets the load policy of the first launch of pacg_convolve, as the autotuner
  * would: non-temporal loads of x when `on` is nonzero (effective where that
- * launch is an A^T pass, that is where a convolution has two or more launches).
- * Drops a captured graph. */
+ * launch is an A^T pass kernel on the VALU, that is where a convolution has two
+ * or more launches and its first pass has no MFMA stage).  Drops a captured
+ * graph. */
 int pacg_plan_stream_input(pacg_plan* plan, int on);
 
 /* Timing aid: runs `reps` convolutions with a HIP event before every kernel

@@ -30,6 +30,24 @@ def test_mfma_plan_one_pass_per_axis(gpu):
     assert mf.launches_per_convolve() == va.launches_per_convolve() == 5, mf.describe()
 
 
+def test_mfma_plan_autotune_keeps_one_load_policy(gpu, oracle):
+    """MFMA pass kernels have no streamed-input form: the autotuner leaves the first launch's load
+    policy alone on an MFMA plan whose first pass has an MFMA stage, and the tuned plan keeps parity."""
+    dims, scheds = (512, 256), [[8, 8, 8], [8, 8, 4]]  # first launch: axis 1, one MFMA pass 8-8-4
+    x = rnd(35, dims)
+    k = rnd(36, dims)
+    pl = gpu.Plan(dims, schedules=scheds, precision=gpu.PREC_F32, mfma=True)
+    assert pl.describe().split("]")[0].count("mfma") == 1, pl.describe()
+    pl.set_kernel(k)
+    dx = gpu.DeviceComplex.from_numpy(x, np.float32)
+    dy = pl.alloc()
+    pl.autotune(dx, dy, 2)
+    assert "x:nt" not in pl.describe(), pl.describe()
+    pl.convolve(dx, dy)
+    y_ref, _ = oracle.pa_convolution(x, k, scheds)
+    assert rel_l2(dy.numpy(), y_ref) < TOL32
+
+
 def rnd(seed, shape):
     g = np.random.default_rng(seed)
     return g.uniform(-1, 1, shape) + 1j * g.uniform(-1, 1, shape)
