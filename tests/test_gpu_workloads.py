@@ -111,3 +111,36 @@ def test_3d_256cubed_shift_and_linearity(gpu, synth):
     pl.convolve(gpu.DeviceComplex.from_numpy(2.0 * x - x2, np.float32), dy)
     y12 = dy.numpy()
     assert rel_l2(y12, 2.0 * np.roll(x, shift, axis=(0, 1, 2)) - y2) < 2e-6
+
+
+@pytest.mark.parametrize("prec", [1, 0])
+def test_1d_2e28_shifted_delta(gpu, prec):
+    """2^28 points, the largest size of the reference's published per-element costs
+    (per-element-butterfly-cost-plot.py:17-34), at full size: a kernel that is a delta at `shift`
+    rolls x by `shift` (a closed form that fixes every index of the convolution), FP32 and FP64
+    plans with their default schedules.  Real and imaginary parts are checked apart, so the host
+    never holds a complex copy of the 2^28-point arrays."""
+    n = 1 << 28
+    shift = 123_456_789
+    dt = np.float32 if prec else np.float64
+    g = np.random.default_rng(280 + prec)
+    xr = g.standard_normal(n, dtype=np.float32).astype(dt, copy=False)
+    xi = g.standard_normal(n, dtype=np.float32).astype(dt, copy=False)
+    pl = gpu.Plan((n,), precision=prec, spectrum_precision=prec)
+    k = np.zeros(n)
+    k[shift] = 1.0
+    pl.set_kernel(k)
+    del k
+    dx = gpu.DeviceComplex((n,), dt)
+    dx.re.upload(xr)
+    dx.im.upload(xi)
+    dy = pl.alloc()
+    pl.convolve(dx, dy)
+    num = den = 0.0
+    for part, xp in ((dy.re, xr), (dy.im, xi)):
+        yp = part.download().astype(np.float64)
+        ref = np.roll(xp, shift).astype(np.float64)
+        num += float(np.dot(yp - ref, yp - ref))
+        den += float(np.dot(ref, ref))
+        del yp, ref
+    assert (num / den) ** 0.5 < (2e-6 if prec else 1e-12), ((num / den) ** 0.5, pl.describe())
