@@ -701,6 +701,7 @@ def bench_batched(pacg, synth, steps, warmup):
             "gflops": flops / (ms_batch * 1e-3) / 1e9,
             "variants": {int(m): pacg.Batch.schedule(int(m), b.variant(int(m))) for m in range(8, 15)},
             "autotuned_lengths": changed,
+            "dealing": ("longest processing time first", "slices of the shape-ordered items")[b.dealing()],
             "direct_launch_ms_per_batch": ms_direct,
             "direct_launch_frac": b_alg / (ms_direct * 1e-3) / 1e9 / HBM_PEAK_GBS,
             "fp64_per_length_ms": ms64, "speedup_vs_fp64_per_length": ms64 / ms_batch,

@@ -86,6 +86,8 @@ def _load():
         getattr(L, nm).argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, _vp]
     L.pacg_batch_autotune.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, _vp, C.c_int]
     L.pacg_batch_set_variant.argtypes = [_vp, C.c_int, C.c_int]
+    L.pacg_batch_set_dealing.argtypes = [_vp, C.c_int]
+    L.pacg_batch_get_dealing.argtypes = [_vp]
     L.pacg_batch_get_variant.argtypes = [_vp, C.c_int]
     L.pacg_batch_num_variants.argtypes = [C.c_int]
     L.pacg_batch_schedule.argtypes = [C.c_int, C.c_int, _ip, C.c_int]
@@ -460,6 +462,15 @@ class Batch:
 
     def variant(self, log2n: int) -> int:
         return int(lib.pacg_batch_get_variant(self.ptr, int(log2n)))
+
+    DEAL_LPT, DEAL_SLICES = 0, 1
+
+    def set_dealing(self, dealing: int):
+        """Items to workgroups: longest processing time first (0) or contiguous slices (1)."""
+        _check(lib.pacg_batch_set_dealing(self.ptr, int(dealing)), "pacg_batch_set_dealing")
+
+    def dealing(self) -> int:
+        return int(lib.pacg_batch_get_dealing(self.ptr))
 
     @staticmethod
     def num_variants(log2n: int) -> int:

@@ -9,7 +9,9 @@
  * pairs of one length n (TPL threads per pair).  Items are dealt to the
  * workgroups longest processing time first (each to the least loaded one),
  * then each workgroup's items are ordered by shape, so that it meets runs
- * of one shape and can load the next item of a run ahead.
+ * of one shape and can load the next item of a run ahead; or, where the
+ * autotuner finds it faster, in contiguous slices of the shape-ordered list
+ * (one or two shapes per workgroup, fewer run starts).
  */
 #define _GNU_SOURCE
 #include <stdlib.h>
@@ -32,6 +34,7 @@ struct pacg_batch {
   long long total;
   int* order;           /* sorted position -> input index */
   int variant[LMAX + 1];
+  int dealing;          /* PACG_DEAL_LPT or PACG_DEAL_SLICES */
   int blocks;           /* workgroups of the persistent launch */
   void* stream;
   int own_stream;
@@ -79,6 +82,11 @@ static int per_item(int m, int v) {
 static double item_cost(int m, int v) {
   const pacg_batch_variant_t* r = variant_row(m, v);
   return r ? (double)r->cost : 1.0;
+}
+/* the same in a batch of mixed lengths (the slice dealing's costs) */
+static double item_cost_mixed(int m, int v) {
+  const pacg_batch_variant_t* r = variant_row(m, v);
+  return r ? (double)r->cost_mixed : 1.0;
 }
 
 typedef struct {
@@ -140,6 +148,41 @@ static int deal_items(const pacg_batch_item_t* items, int n, int w, pacg_batch_i
   return PACG_OK;
 }
 
+/* Time (0.1 us) a workgroup spends each time its list starts a new shape: the
+ * run's first k is loaded before its loop, not behind the previous item's
+ * phases (fitted with the mixed-batch item costs). */
+#define RUN_COST 36.0
+
+/* The slice dealing: items (built longest first, one shape after another) go
+ * to w workgroups as contiguous slices of the built order, so that a workgroup
+ * meets one shape, or two at a slice boundary.  Workgroup j takes the items
+ * whose running mixed-batch cost (plus RUN_COST per slice) falls into its
+ * share j / w .. (j + 1) / w of the total, judged at each item's middle; every
+ * workgroup gets an item.  Balanced only as well as the costs are right, so
+ * the autotuner keeps it only where it measures faster. */
+static int deal_slices(const pacg_batch_item_t* items, int n, int w, pacg_batch_item_t* out, uint32_t* wg_first) {
+  double total = RUN_COST * w;
+  for (int i = 0; i < n; ++i) total += item_cost_mixed(items[i].log2n, items[i].variant);
+  const double share = total / w;
+  double acc = RUN_COST;
+  int j = 0;
+  for (int k = 0; k <= w; ++k) wg_first[k] = 0;
+  for (int i = 0; i < n; ++i) {
+    const double c = item_cost_mixed(items[i].log2n, items[i].variant);
+    /* next workgroup once this one has its share, or when the items left are
+     * only enough for one per workgroup left */
+    if (j < w - 1 && i > 0 && (acc + 0.5 * c > (j + 1) * share || n - i <= w - 1 - j)) {
+      ++j;
+      acc += RUN_COST;
+    }
+    wg_first[j + 1]++;
+    acc += c;
+    out[i] = items[i];
+  }
+  for (int k = 0; k < w; ++k) wg_first[k + 1] += wg_first[k];
+  return PACG_OK;
+}
+
 /* items for the lengths whose mask bit is set, dealt to the workgroups; uploads them */
 static int build_items(pacg_batch* b, unsigned mask) {
   int n = 0;
@@ -171,7 +214,9 @@ static int build_items(pacg_batch* b, unsigned mask) {
   }
   int w = n < b->blocks ? n : b->blocks;
   int rc = PACG_OK;
-  if (n > 0) rc = deal_items(items, n, w, dealt, wg_first);
+  if (n > 0)
+    rc = b->dealing == PACG_DEAL_SLICES ? deal_slices(items, n, w, dealt, wg_first)
+                                        : deal_items(items, n, w, dealt, wg_first);
   if (!rc && n > 0 &&
       (pacg_memcpy_h2d(b->d_items, dealt, (size_t)n * sizeof(pacg_batch_item_t), b->stream) ||
        pacg_memcpy_h2d(b->d_wg_first, wg_first, (size_t)(w + 1) * sizeof(uint32_t), b->stream)))
@@ -297,6 +342,23 @@ int pacg_batch_get_variant(const pacg_batch* b, int log2n) {
   return b->variant[log2n];
 }
 
+int pacg_batch_set_dealing(pacg_batch* b, int dealing) {
+  if (!b || (dealing != PACG_DEAL_LPT && dealing != PACG_DEAL_SLICES))
+    return pacg_fail(PACG_ERR_ARG, "pacg_batch_set_dealing: bad argument");
+  b->dealing = dealing;
+  int rc = build_items(b, ~0u);
+  if (b->graph) {
+    pacg_graph_destroy(b->graph);
+    b->graph = NULL;
+  }
+  return rc;
+}
+
+int pacg_batch_get_dealing(const pacg_batch* b) {
+  if (!b) return -pacg_fail(PACG_ERR_ARG, "pacg_batch_get_dealing: NULL batch");
+  return b->dealing;
+}
+
 int pacg_batch_set_variant(pacg_batch* b, int log2n, int variant) {
   if (!b || log2n < LMIN || log2n > LMAX || !variant_row(log2n, variant))
     return pacg_fail(PACG_ERR_ARG, "no batched variant %d for length 2^%d", variant, log2n);
@@ -357,16 +419,12 @@ static int time_batch(pacg_batch* b, const float* xr, const float* xi, const flo
  * workgroups beside the others.  Two sweeps, each variant keeping its faster
  * measurement; a length keeps its current variant unless another is faster by
  * more than 1 %. */
-int pacg_batch_autotune(pacg_batch* b, const float* xr, const float* xi, const float* kr, const float* ki, float* yr,
-                        float* yi, int reps) {
-  if (!b || !xr || !xi || !kr || !ki || !yr || !yi || reps < 1) return -pacg_fail(PACG_ERR_ARG, "bad argument");
-  void *e0 = NULL, *e1 = NULL;
-  if (pacg_event_create(&e0) || pacg_event_create(&e1)) {
-    if (e0) pacg_event_destroy(e0);
-    return -pacg_fail(PACG_ERR_HIP, "event creation failed");
-  }
-  int start[LMAX + 1];
-  for (int m = 0; m <= LMAX; ++m) start[m] = b->variant[m];
+/* Coordinate descent over the variants of the lengths present, with the
+ * batch's current dealing: two sweeps, each variant timed twice on the whole
+ * batch (its faster timing kept), a variant replacing the current one only
+ * when more than 1 % faster. */
+static int sweep_variants(pacg_batch* b, const float* xr, const float* xi, const float* kr, const float* ki, float* yr,
+                          float* yi, int reps, void* e0, void* e1) {
   int rc = PACG_OK;
   for (int sweep = 0; sweep < 2 && !rc; ++sweep) {
     for (int m = LMAX; m >= LMIN && !rc; --m) {
@@ -393,8 +451,49 @@ int pacg_batch_autotune(pacg_batch* b, const float* xr, const float* xi, const f
       b->variant[m] = best;
     }
   }
+  return rc;
+}
+
+/* The variants for each dealing in turn (sweeps from the batch's starting
+ * variants), each result timed four times (its fastest kept); the slices
+ * replace the default dealing, longest processing time first, only when more
+ * than 1 % faster.  The dealing and the variants interact: a variant with
+ * smaller items balances the default dealing, the slices need fewer shapes
+ * per workgroup, so they are searched together. */
+int pacg_batch_autotune(pacg_batch* b, const float* xr, const float* xi, const float* kr, const float* ki, float* yr,
+                        float* yi, int reps) {
+  if (!b || !xr || !xi || !kr || !ki || !yr || !yi || reps < 1) return -pacg_fail(PACG_ERR_ARG, "bad argument");
+  void *e0 = NULL, *e1 = NULL;
+  if (pacg_event_create(&e0) || pacg_event_create(&e1)) {
+    if (e0) pacg_event_destroy(e0);
+    return -pacg_fail(PACG_ERR_HIP, "event creation failed");
+  }
+  int start[LMAX + 1], found[2][LMAX + 1];
+  for (int m = 0; m <= LMAX; ++m) start[m] = b->variant[m];
+  const int had = b->dealing;
+  float dms[2] = {1e30f, 1e30f};
+  int rc = PACG_OK;
+  for (int d = 0; d < 2 && !rc; ++d) {
+    b->dealing = d;
+    for (int m = 0; m <= LMAX; ++m) b->variant[m] = start[m];
+    if ((rc = sweep_variants(b, xr, xi, kr, ki, yr, yi, reps, e0, e1))) break;
+    for (int m = 0; m <= LMAX; ++m) found[d][m] = b->variant[m];
+    if ((rc = build_items(b, ~0u))) break;
+    for (int t = 0; t < 4 && !rc; ++t) {
+      float ms = 0.0f;
+      if ((rc = time_batch(b, xr, xi, kr, ki, yr, yi, reps, e0, e1, &ms))) break;
+      if (ms < dms[d]) dms[d] = ms;
+    }
+  }
   pacg_event_destroy(e0);
   pacg_event_destroy(e1);
+  if (rc) { /* leave the batch as it was */
+    b->dealing = had;
+    for (int m = 0; m <= LMAX; ++m) b->variant[m] = start[m];
+  } else {
+    b->dealing = dms[PACG_DEAL_SLICES] < 0.99f * dms[PACG_DEAL_LPT] ? PACG_DEAL_SLICES : PACG_DEAL_LPT;
+    for (int m = 0; m <= LMAX; ++m) b->variant[m] = found[b->dealing][m];
+  }
   int rc2 = build_items(b, ~0u);
   if (b->graph) {
     pacg_graph_destroy(b->graph);

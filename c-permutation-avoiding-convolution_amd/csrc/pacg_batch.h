@@ -38,6 +38,7 @@ typedef struct {
   int radix[7];                      /* stage 0 first */
   int tpl;                           /* threads per convolution */
   int cost;                          /* time of one item on one workgroup, 0.1 us (MI355X) */
+  int cost_mixed;                    /* the same in a batch of mixed lengths (fitted), for the slice dealing */
 } pacg_batch_variant_t;
 
 extern const pacg_batch_variant_t pacg_batch_variants[];

@@ -388,27 +388,30 @@ __device__ __forceinline__ uint32_t batch_run(const pacg_batch_args_t& a, uint32
 // fewer phase twiddles); at 2^13 and 2^14 its phases below the top span 1024
 // points, one wavefront's share, so only the exchange with the top phase
 // needs the workgroup barrier.  The other variants are alternatives for the
-// autotuner (8 points per thread where listed).  The last column is the
-// time of one item on one workgroup in 0.1 us, measured on MI355X with
-// batches of one length (2^24 points): the host deals items to workgroups by
-// it.  Kept in step with pacg_batch_variants[] below.
+// autotuner (8 points per thread where listed).  The last two columns are
+// the time of one item on one workgroup in 0.1 us on MI355X: measured with
+// batches of one length (2^24 points), the costs of the default dealing; and
+// fitted (least squares, with a cost per run of one shape) to per-workgroup
+// timings of batches of mixed lengths 2^U{8..14}, where short items run beside
+// long ones and cost relatively less, the costs of the slice dealing.  Kept in
+// step with pacg_batch_variants[] below.
 #define PACG_BATCH_VARIANTS(X)                                                \
-  X(8, 0, 4, 0xAAu, 2, 0x31u, 201)         /* 4 4 | 4 4                */  \
-  X(8, 1, 3, 0x2Fu, 3, 0x210u, 120)        /* 8 | 8 | 4                */  \
-  X(9, 0, 5, 0x1AAu, 3, 0x431u, 197)       /* 4 4 | 4 4 | 2            */  \
-  X(9, 1, 3, 0x3Fu, 3, 0x210u, 124)        /* 8 | 8 | 8                */  \
-  X(10, 0, 5, 0x2AAu, 3, 0x431u, 195)      /* 4 4 | 4 4 | 4            */  \
-  X(10, 1, 4, 0xAFu, 4, 0x3210u, 147)      /* 8 | 8 | 4 | 4            */  \
-  X(11, 0, 5, 0x3AAu, 3, 0x431u, 249)      /* 4 4 | 4 4 | 8            */  \
-  X(11, 1, 4, 0xBFu, 4, 0x3210u, 143)      /* 8 | 8 | 8 | 4            */  \
-  X(12, 0, 6, 0xAAAu, 3, 0x531u, 251)      /* 4 4 | 4 4 | 4 4          */  \
-  X(12, 1, 4, 0xFFu, 4, 0x3210u, 144)      /* 8 | 8 | 8 | 8            */  \
-  X(13, 0, 6, 0xEAAu, 4, 0x5431u, 298)     /* 4 4 | 4 4 | 4 | 8        */  \
-  X(13, 1, 7, 0x1AAAu, 4, 0x6531u, 305)    /* 4 4 | 4 4 | 4 4 | 2      */  \
-  X(13, 2, 5, 0x1FFu, 5, 0x43210u, 182)    /* 8 | 8 | 8 | 8 | 2        */  \
-  X(14, 0, 7, 0x2AAAu, 4, 0x6431u, 290)    /* 4 4 | 4 4 | 4 | 4 4      */  \
-  X(14, 1, 7, 0x2AAAu, 4, 0x6531u, 305)    /* 4 4 | 4 4 | 4 4 | 4      */  \
-  X(14, 2, 7, 0x2AAAu, 4, 0x6420u, 288)    /* 4 | 4 4 | 4 4 | 4 4      */
+  X(8, 0, 4, 0xAAu, 2, 0x31u, 201, 135)         /* 4 4 | 4 4                */  \
+  X(8, 1, 3, 0x2Fu, 3, 0x210u, 120, 54)         /* 8 | 8 | 4                */  \
+  X(9, 0, 5, 0x1AAu, 3, 0x431u, 197, 137)       /* 4 4 | 4 4 | 2            */  \
+  X(9, 1, 3, 0x3Fu, 3, 0x210u, 124, 71)         /* 8 | 8 | 8                */  \
+  X(10, 0, 5, 0x2AAu, 3, 0x431u, 195, 137)      /* 4 4 | 4 4 | 4            */  \
+  X(10, 1, 4, 0xAFu, 4, 0x3210u, 147, 69)       /* 8 | 8 | 4 | 4            */  \
+  X(11, 0, 5, 0x3AAu, 3, 0x431u, 249, 169)      /* 4 4 | 4 4 | 8            */  \
+  X(11, 1, 4, 0xBFu, 4, 0x3210u, 143, 121)      /* 8 | 8 | 8 | 4            */  \
+  X(12, 0, 6, 0xAAAu, 3, 0x531u, 251, 171)      /* 4 4 | 4 4 | 4 4          */  \
+  X(12, 1, 4, 0xFFu, 4, 0x3210u, 144, 114)      /* 8 | 8 | 8 | 8            */  \
+  X(13, 0, 6, 0xEAAu, 4, 0x5431u, 298, 228)     /* 4 4 | 4 4 | 4 | 8        */  \
+  X(13, 1, 7, 0x1AAAu, 4, 0x6531u, 305, 218)    /* 4 4 | 4 4 | 4 4 | 2      */  \
+  X(13, 2, 5, 0x1FFu, 5, 0x43210u, 182, 127)    /* 8 | 8 | 8 | 8 | 2        */  \
+  X(14, 0, 7, 0x2AAAu, 4, 0x6431u, 290, 205)    /* 4 4 | 4 4 | 4 | 4 4      */  \
+  X(14, 1, 7, 0x2AAAu, 4, 0x6531u, 305, 228)    /* 4 4 | 4 4 | 4 4 | 4      */  \
+  X(14, 2, 7, 0x2AAAu, 4, 0x6420u, 288, 208)    /* 4 | 4 4 | 4 4 | 4 4      */
 
 __global__ void __launch_bounds__(BT) batch_kernel(const pacg_batch_args_t a) {
   __shared__ float2 lds[BLD];
@@ -427,7 +430,7 @@ __global__ void __launch_bounds__(BT) batch_kernel(const pacg_batch_args_t a) {
     const pacg_batch_item_t it = a.items[i];
     const int key = (int)it.log2n * 4 + (int)it.variant;
     switch (key) {
-#define PACG_CASE(M, V, NS, SC, NPH, PC, COST)                           \
+#define PACG_CASE(M, V, NS, SC, NPH, PC, COST, MIXED)                   \
   case M * 4 + V:                                                       \
     static_assert(Shape<NS, SC, NPH, PC>::LGP == M, "variant length"); \
     i = batch_run<Shape<NS, SC, NPH, PC>>(a, i, end, lds, twh, twl, first); \
@@ -449,12 +452,12 @@ extern "C" {
 // Stage schedules of the variants (stage 0 first), for the host and tests.
 #define PACG_SH(NS, SC, NPH, PC) pacg::Shape<NS, SC, NPH, PC>
 #define PACG_RHO(NS, SC, NPH, PC, G) (NS > G ? PACG_SH(NS, SC, NPH, PC)::rho(G) : 0)
-#define PACG_ROW(M, V, NS, SC, NPH, PC, COST)                                                               \
+#define PACG_ROW(M, V, NS, SC, NPH, PC, COST, MIXED)                                                     \
   {M, V, NS,                                                                                             \
    {PACG_RHO(NS, SC, NPH, PC, 0), PACG_RHO(NS, SC, NPH, PC, 1), PACG_RHO(NS, SC, NPH, PC, 2),            \
     PACG_RHO(NS, SC, NPH, PC, 3), PACG_RHO(NS, SC, NPH, PC, 4), PACG_RHO(NS, SC, NPH, PC, 5),            \
     PACG_RHO(NS, SC, NPH, PC, 6)},                                                                       \
-   PACG_SH(NS, SC, NPH, PC)::TPL, COST},
+   PACG_SH(NS, SC, NPH, PC)::TPL, COST, MIXED},
 const pacg_batch_variant_t pacg_batch_variants[] = {PACG_BATCH_VARIANTS(PACG_ROW)};
 #undef PACG_RHO
 #undef PACG_SH

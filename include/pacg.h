@@ -193,14 +193,25 @@ void* pacg_batch_stream(const pacg_batch* batch);
 int pacg_batch_items(const pacg_batch* batch);
 int pacg_batch_convolve(pacg_batch* batch, const float* x_re, const float* x_im, const float* k_re, const float* k_im,
                         float* y_re, float* y_im);
-/* Chooses a compiled schedule per length present: each is timed on the whole
- * batch (these buffers), the other lengths at their current schedules, over
- * two sweeps, and the fastest is kept.  Returns how many lengths changed
- * schedule, or -error.  Drops a captured graph. */
+/* Chooses a compiled schedule per length present and the dealing of items to
+ * workgroups (below): for each dealing, each length's schedules are timed on
+ * the whole batch (these buffers), the other lengths at their current
+ * schedules, over two sweeps, keeping the fastest; then the faster dealing
+ * with its schedules is kept.  Returns how many lengths changed schedule, or
+ * -error (the batch then keeps its schedules and dealing).  Drops a captured
+ * graph. */
 int pacg_batch_autotune(pacg_batch* batch, const float* x_re, const float* x_im, const float* k_re,
                         const float* k_im, float* y_re, float* y_im, int reps);
 int pacg_batch_set_variant(pacg_batch* batch, int log2n, int variant);
 int pacg_batch_get_variant(const pacg_batch* batch, int log2n);
+/* Dealing of the work items to the workgroups of the launch: longest
+ * processing time first (the default), or contiguous slices of the
+ * shape-ordered item list (one or two shapes per workgroup: fewer run starts,
+ * balanced by fitted costs).  Results do not depend on it.  Set drops a
+ * captured graph; get returns the dealing or -error. */
+enum { PACG_DEAL_LPT = 0, PACG_DEAL_SLICES = 1 };
+int pacg_batch_set_dealing(pacg_batch* batch, int dealing);
+int pacg_batch_get_dealing(const pacg_batch* batch);
 int pacg_batch_num_variants(int log2n);
 /* Stage radices (stage 0 first) of a variant; returns the stage count or -error. */
 int pacg_batch_schedule(int log2n, int variant, int* radix, int max_stages);

@@ -152,9 +152,27 @@ static void check_batch(pacg_batch* b, const char* what) {
     if (t > worst_wg) worst_wg = t;
   }
   for (int p = 0; p < b->count; ++p) CHECK(seen[p] == 1, "%s: position %d covered %d times", what, p, seen[p]);
-  /* longest processing time first: within one item of the mean */
-  CHECK(worst_wg <= total / b->n_wg + worst_item + 1e-9, "%s: makespan %.0f, mean %.0f, item %.0f", what, worst_wg,
-        total / b->n_wg, worst_item);
+  if (b->dealing == PACG_DEAL_LPT) {
+    /* longest processing time first: within one item of the mean */
+    CHECK(worst_wg <= total / b->n_wg + worst_item + 1e-9, "%s: makespan %.0f, mean %.0f, item %.0f", what,
+          worst_wg, total / b->n_wg, worst_item);
+  } else {
+    /* slices of the built order: a workgroup's mixed-batch cost (plus a run
+     * cost per slice) is within two items and a run cost of its share */
+    double tm = 0.0, wm = 0.0, im = 0.0;
+    for (int w = 0; w < b->n_wg; ++w) {
+      double t = RUN_COST;
+      for (uint32_t i = wf[w]; i < wf[w + 1]; ++i) {
+        const double c = item_cost_mixed(it[i].log2n, it[i].variant);
+        t += c;
+        if (c > im) im = c;
+      }
+      tm += t;
+      if (t > wm) wm = t;
+    }
+    CHECK(wm <= tm / b->n_wg + 2.0 * im + RUN_COST + 1e-9, "%s: slice makespan %.0f, mean %.0f, item %.0f", what,
+          wm, tm / b->n_wg, im);
+  }
   free(seen);
 }
 
@@ -182,6 +200,11 @@ static void batches(void) {
           check_batch(b, what);
           CHECK(pacg_batch_set_variant(b, len, 0) == 0, "set variant 0");
         }
+      CHECK(pacg_batch_set_dealing(b, PACG_DEAL_SLICES) == 0 && pacg_batch_get_dealing(b) == PACG_DEAL_SLICES,
+            "set dealing: %s", g_err);
+      check_batch(b, what);
+      CHECK(pacg_batch_set_dealing(b, PACG_DEAL_LPT) == 0, "set dealing back");
+      CHECK(pacg_batch_set_dealing(b, 7) == PACG_ERR_ARG, "dealing 7 accepted");
       float fake = 0.0f;
       CHECK(pacg_batch_autotune(b, &fake, &fake, &fake, &fake, &fake, &fake, 2) >= 0, "autotune: %s", g_err);
       check_batch(b, what);

@@ -118,6 +118,39 @@ def test_ragged_items_of_8192_points(gpu, oracle):
     check(b, xs, ks, dy, oracle, {m: gpu.Batch.schedule(m, b.variant(m)) for m in range(8, 15)})
 
 
+def test_both_dealings_give_identical_results(gpu, synth):
+    """Items dealt longest processing time first or in slices of the shape-ordered list: each
+    convolution is the same item computed by the same kernel code, only on another workgroup, so
+    the outputs are bit-identical (500 pairs of the benchmark's length distribution, every
+    variant set the autotuner may choose here); the autotuner keeps one of the two dealings."""
+    xs, ks = synth.batched_inputs(4, 500)
+    ms = np.array([int(np.log2(len(x))) for x in xs], dtype=np.int32)
+    b = gpu.Batch(ms)
+    dx = gpu.DeviceComplex.from_numpy(b.pack(xs), np.float32)
+    dk = gpu.DeviceComplex.from_numpy(b.pack(ks), np.float32)
+    dy = b.alloc()
+    assert b.dealing() == gpu.Batch.DEAL_LPT
+    for combo in ([0] * 7, [1, 1, 1, 1, 1, 2, 2]):
+        for m, v in zip(range(8, 15), combo):
+            b.set_variant(m, v)
+        out = []
+        for dealing in (gpu.Batch.DEAL_LPT, gpu.Batch.DEAL_SLICES):
+            b.set_dealing(dealing)
+            assert b.dealing() == dealing
+            dy.upload(np.zeros(int(b.total), complex))
+            b.convolve(dx, dk, dy)
+            b.sync()
+            out.append(dy.numpy())
+        assert np.array_equal(out[0], out[1]), combo
+    ys = b.unpack(out[0])
+    for x, k, y in zip(xs[:40], ks[:40], ys[:40]):
+        assert rel_l2(y, np.fft.ifft(np.fft.fft(x) * np.fft.fft(k))) < TOL32
+    b.autotune(dx, dk, dy, 2)
+    assert b.dealing() in (gpu.Batch.DEAL_LPT, gpu.Batch.DEAL_SLICES)
+    with pytest.raises(gpu.PacgError):
+        b.set_dealing(2)
+
+
 def test_full_size_batch_of_4096_pairs(gpu, synth):
     """The benchmark batch itself (4096 pairs, lengths 2^U{8..14}, seed 4), autotuned and replayed
     from its graph: every pair of every length against numpy.fft in complex128, per pair and over the

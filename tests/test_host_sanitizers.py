@@ -16,7 +16,7 @@ CSRC = os.path.join(PKG_DIR, "csrc")
 
 class _Variant(C.Structure):
     _fields_ = [("log2n", C.c_int), ("variant", C.c_int), ("nstages", C.c_int), ("radix", C.c_int * 7),
-                ("tpl", C.c_int), ("cost", C.c_int)]
+                ("tpl", C.c_int), ("cost", C.c_int), ("cost_mixed", C.c_int)]
 
 
 def test_host_layer_under_asan_and_ubsan(pacg, tmp_path):
@@ -28,8 +28,9 @@ def test_host_layer_under_asan_and_ubsan(pacg, tmp_path):
     lines = ["/* written by tests/test_host_sanitizers.py from the built library */",
              "const pacg_batch_variant_t pacg_batch_variants[] = {"]
     for v in rows:
-        lines.append("  {%d, %d, %d, {%s}, %d, %d}," % (v.log2n, v.variant, v.nstages, ", ".join(str(r) for r in v.radix),
-                                                      v.tpl, v.cost))
+        lines.append("  {%d, %d, %d, {%s}, %d, %d, %d}," % (v.log2n, v.variant, v.nstages,
+                                                          ", ".join(str(r) for r in v.radix), v.tpl, v.cost,
+                                                          v.cost_mixed))
     lines += ["};", "const int pacg_num_batch_variants = %d;" % n]
     (tmp_path / "pacg_batch_variants.h").write_text("\n".join(lines) + "\n")
     exe = str(tmp_path / "host_sanitize")
