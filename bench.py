@@ -764,7 +764,9 @@ def main():
         for _ in range(a.steps):
             b.replay()
         b.sync()
-        print(json.dumps({"profile_batched": True, "steps": a.steps, "points": int(b.total)}))
+        print(json.dumps({"profile_batched": True, "steps": a.steps, "points": int(b.total),
+                          "dealing": ("longest processing time first", "slices of the shape-ordered items")[b.dealing()],
+                          "variants": {m: pacg.Batch.schedule(m, b.variant(m)) for m in range(8, 15)}}))
         return
 
     if a.profile_fp64_2d:
