@@ -440,7 +440,26 @@ __device__ __forceinline__ int row_flip(int s) {
     return __builtin_parity((unsigned)s & 0x14u) | (__builtin_parity((unsigned)s & 0x28u) << 1);
 }
 
-enum { LM_PAD = 0, LM_ROWS = 1, LM_ROWSWZ = 2, LM_LINESWZ = 3, LM_ROWXOR = 4 };
+enum { LM_PAD = 0, LM_ROWS = 1, LM_ROWSWZ = 2, LM_LINESWZ = 3, LM_ROWXOR = 4, LM_QUADSWZ = 5 };
+
+// Slot of point s in a contiguous FP64 line of (re, im) pairs (16-byte slots,
+// eight to 32 banks of 4 bytes) whose phases below the top are single radix-4
+// stages: bits 0-3 of s become s0^s4, s1^s3, s2^s4, s3^s5.  Consecutive
+// threads then hold points 4 apart (phase 0), in runs of 4 that are 16 apart
+// (phase 1) or in runs of 16 or more (the others), and in every one of these
+// layouts each aligned group of 8 or 16 lanes meets 8 or 16 different slots
+// modulo 8 or 16, where one pad slot per 8 points leaves 2-way conflicts.
+// Each bit is XOR-ed only with bits above it, so the map is a bijection; it is
+// linear, so the slot of pos(tl, 0, 0) | pos(0, c, r) is the XOR of their slots.
+__device__ __forceinline__ constexpr int quad_swz(int s) {
+  return s ^ (((s >> 4) & 1) | (((s >> 3) & 1) << 1) | (((s >> 4) & 1) << 2) | (((s >> 5) & 1) << 3));
+}
+template <class SH>
+constexpr bool radix4_phases() {
+  for (int p = 0; p + 1 < SH::NP; ++p)
+    if (SH::plo(p) != SH::phi(p) || SH::lg(SH::plo(p)) != 2) return false;
+  return SH::NP > 1;
+}
 
 // Row of point s in a strided FP64 tile of long lines ([P][LINES], no
 // padding): the low NB bits of s are XOR-ed with every NB-bit group of s above
@@ -492,12 +511,16 @@ __global__ void __launch_bounds__(LINES* SH::TPL) __attribute__((amdgpu_waves_pe
   //              points: [LINES][P] with lds_swz inside each line.
   // FP64 strided tiles of lines of 2048+ points (a few lines wide, so that a
   // tile fits in LDS at all) use LM_ROWXOR: [P][LINES] with the row folded by
-  // row_fold, no padding.  Other tiles and FP64 keep LM_PAD: strided
-  // [P][LINES+1], contiguous [LINES][P + P/8 + 1] with one pad slot per 8 points.
+  // row_fold, no padding.  FP64 contiguous lines of 2048+ points, 8 points
+  // per thread, in single radix-4 phases below the top use LM_QUADSWZ:
+  // [LINES][P] with quad_swz inside each line.  Other tiles keep LM_PAD:
+  // strided [P][LINES+1], contiguous [LINES][P + P/8 + 1] with one pad slot
+  // per 8 points.
   constexpr bool F32 = sizeof(T) == 4;
   constexpr bool LONG_S = !F32 && LAYOUT == PACG_L_STRIDED && P >= 2048;
-  constexpr int LMODE = LONG_S                      ? LM_ROWXOR
-                        : !F32                      ? LM_PAD
+  constexpr int LMODE = LONG_S ? LM_ROWXOR
+                        : !F32  ? ((LAYOUT == PACG_L_CONTIG && P >= 2048 && E == 8 && radix4_phases<SH>()) ? LM_QUADSWZ
+                                                                                                          : LM_PAD)
                         : LAYOUT == PACG_L_STRIDED ? (LINES >= 32                  ? LM_ROWS
                                                       : (LINES == 16 || LINES == 8) ? LM_ROWSWZ
                                                                                     : LM_PAD)
@@ -505,7 +528,7 @@ __global__ void __launch_bounds__(LINES* SH::TPL) __attribute__((amdgpu_waves_pe
                                                               : LM_PAD;
   constexpr int XNB = LINES >= 8 ? 4 : 3;  // fold width of LM_ROWXOR
   constexpr int LDJ = (LMODE == LM_ROWS || LMODE == LM_ROWSWZ || LMODE == LM_ROWXOR) ? LINES : LINES + 1;
-  constexpr int LDP = LMODE == LM_LINESWZ ? P : P + P / 8 + 1;
+  constexpr int LDP = (LMODE == LM_LINESWZ || LMODE == LM_QUADSWZ) ? P : P + P / 8 + 1;
   constexpr int LDS_N = (LAYOUT == PACG_L_STRIDED) ? P * LDJ : LINES * LDP;
   constexpr bool NEED_X = NP > 1;
   // FP64 contiguous lines of 2048+ points with 16 points per thread exchange
@@ -530,9 +553,15 @@ __global__ void __launch_bounds__(LINES* SH::TPL) __attribute__((amdgpu_waves_pe
   __shared__ T xi[NEED_X && !SPLIT && !PAIRS ? LDS_N : 1];
   __shared__ V2T xv[NEED_X && PAIRS ? LDS_N : 1];
   // w_P in LDS: the whole table up to 512 points, two levels above
-  // (w_P^p = hi[p >> 6] * lo[p & 63]) so that long lines keep LDS for data
+  // (w_P^p = hi[p >> 6] * lo[p & 63]) so that long lines keep LDS for data.
+  // The 64 lo entries sit after the P / 64 hi entries with one pad slot per
+  // 8: the phases whose twiddle exponents step by 8 read lo entries 8 apart,
+  // which would otherwise fall into one bank group (an 8-way conflict with
+  // 16-byte entries).
   constexpr bool TW2 = P > 512;
-  constexpr int TWN = TW2 ? P / 64 + 64 : P;
+  constexpr int TWN = TW2 ? P / 64 + 72 : P;
+  constexpr int TWLO = P / 64;  // first lo slot
+  auto lo_slot = [](int j) PACG_INL { return TWLO + j + (j >> 3); };
   __shared__ T twr[PAIRS ? 1 : TWN], twi[PAIRS ? 1 : TWN];
   __shared__ V2T twv[PAIRS ? TWN : 1];
 
@@ -572,13 +601,15 @@ __global__ void __launch_bounds__(LINES* SH::TPL) __attribute__((amdgpu_waves_pe
   {
     using V2 = typename Vec2<T>::type;
     const V2* tp = reinterpret_cast<const V2*>(a.tw_p);
-    for (int i = tid; i < TWN; i += NT) {
+    constexpr int NTW = TW2 ? P / 64 + 64 : P;  // entries (the pad slots stay unwritten)
+    for (int i = tid; i < NTW; i += NT) {
       const V2 w = tp[TW2 ? (i < P / 64 ? i * 64 : i - P / 64) : i];
+      const int q = TW2 && i >= P / 64 ? lo_slot(i - P / 64) : i;
       if constexpr (PAIRS) {
-        twv[i] = w;
+        twv[q] = w;
       } else {
-        twr[i] = w.x;
-        twi[i] = w.y;
+        twr[q] = w.x;
+        twi[q] = w.y;
       }
     }
   }
@@ -590,7 +621,7 @@ __global__ void __launch_bounds__(LINES* SH::TPL) __attribute__((amdgpu_waves_pe
     constexpr int SER = (LONG_S && E >= 16) ? 2 : 0;
     if constexpr (PAIRS && TW2) {
       phase_twiddle_f<T, SH, PH, CJ, SER>(vr, vi, tl, [&](int idx, T& wr, T& wi) PACG_INL {
-        const V2T h = twv[idx >> 6], l = twv[P / 64 + (idx & 63)];
+        const V2T h = twv[idx >> 6], l = twv[lo_slot(idx & 63)];
         wr = h.x * l.x - h.y * l.y;
         wi = h.x * l.y + h.y * l.x;
       });
@@ -603,7 +634,7 @@ __global__ void __launch_bounds__(LINES* SH::TPL) __attribute__((amdgpu_waves_pe
     } else if constexpr (TW2) {
       phase_twiddle_f<T, SH, PH, CJ, SER>(vr, vi, tl, [&](int idx, T& wr, T& wi) PACG_INL {
         const T hr = twr[idx >> 6], hi = twi[idx >> 6];
-        const T lr = twr[P / 64 + (idx & 63)], li = twi[P / 64 + (idx & 63)];
+        const T lr = twr[lo_slot(idx & 63)], li = twi[lo_slot(idx & 63)];
         wr = hr * lr - hi * li;
         wi = hr * li + hi * lr;
       });
@@ -632,6 +663,8 @@ __global__ void __launch_bounds__(LINES* SH::TPL) __attribute__((amdgpu_waves_pe
       return ((((st ^ row_flip<LINES>(st)) ^ pat) << LGL) | ll) + rest;
     } else if constexpr (LMODE == LM_LINESWZ) {
       return ((ll * P) | lds_swz(st)) ^ lds_swz(sc);
+    } else if constexpr (LMODE == LM_QUADSWZ) {
+      return (ll * P) | (quad_swz(st) ^ quad_swz(sc));
     } else if constexpr (LMODE == LM_ROWXOR) {
       // row_fold is linear and ll < LINES: the constant part is one XOR
       return (row_fold<XNB, SH::LGP>(st) * LINES + ll) ^ (row_fold<XNB, SH::LGP>(sc) * LINES);
