@@ -37,6 +37,9 @@ CASES = [
     # ... on an axis whose inner size (2) is smaller than a tile's line count: one tile spans
     # several outer indices
     ((512, 2048, 2), [[8, 8, 8], [4, 4, 4, 4, 4, 2], [2]], 0),
+    # ... with 4 lines in all: the 8-line tile has 4 lines past the end (they load line 0 and
+    # store nothing)
+    ((2048, 4), [[4, 4, 4, 4, 4, 2], [4]], 0),
 ]
 
 
