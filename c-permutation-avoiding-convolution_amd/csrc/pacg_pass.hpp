@@ -493,8 +493,11 @@ constexpr int pass_min_waves() {
 // instantiation, since a branch on a flag inside one kernel costs registers in
 // both forms): bit 0, the fused pass streams the spectrum with non-temporal
 // loads; bit 1, an A^T pass loads its input non-temporally (the caller's x,
-// which one convolution reads once).
-enum { NT_SPEC = 1, NT_IN = 2 };
+// which one convolution reads once); bit 3, NO_EXT, the pass applies no
+// pass-external twiddle (a.ext == 0) and its kernel leaves that code out: the
+// FP64 strided conj(A) and A tiles of 2048-point lines, which start with it,
+// need 105 instead of 128 VGPRs without it and no longer spill (26 VGPRs).
+enum { NT_SPEC = 1, NT_IN = 2, NO_EXT = 8 };
 template <typename T, typename TS, class SH, int KIND, int LAYOUT, int LINES, int NTM = 0>
 __global__ void __launch_bounds__(LINES* SH::TPL) __attribute__((amdgpu_waves_per_eu(pass_min_waves<T, SH, LAYOUT>())))
     pass_kernel(const pacg_pass_args_t a) {
@@ -938,7 +941,9 @@ __global__ void __launch_bounds__(LINES* SH::TPL) __attribute__((amdgpu_waves_pe
     constexpr int MODE = (KIND == PACG_K_DITC) ? M_DITC : M_DIT;
     load_phase(std::integral_constant<int, 0>{}, inr, ini);
     __syncthreads();
-    if (a.ext) ext_twiddle(MODE == M_DITC);
+    if constexpr ((NTM & NO_EXT) == 0) {
+      if (a.ext) ext_twiddle(MODE == M_DITC);
+    }
     sfor<NP>([&](auto i_) PACG_INL {
       constexpr int PH = decltype(i_)::value;
       if constexpr (PH > 0) exchange(std::integral_constant<int, PH - 1>{}, std::integral_constant<int, PH>{});
@@ -966,6 +971,14 @@ int launch_pass(const pacg_pass_args_t* a, void* stream) {
   if constexpr (KIND == PACG_K_FUSED && sizeof(TS) == 8 && LAYOUT == PACG_L_STRIDED) {
     if (a->spec_nt) {
       hipLaunchKernelGGL((pass_kernel<T, TS, SH, KIND, LAYOUT, LINES, NT_SPEC>), dim3(blocks), dim3(LINES * SH::TPL), 0,
+                         (hipStream_t)stream, *a);
+      return (int)hipGetLastError();
+    }
+  }
+  if constexpr ((KIND == PACG_K_DITC || KIND == PACG_K_DIT) && sizeof(T) == 8 && LAYOUT == PACG_L_STRIDED &&
+                SH::P >= 2048) {
+    if (!a->ext) {
+      hipLaunchKernelGGL((pass_kernel<T, TS, SH, KIND, LAYOUT, LINES, NO_EXT>), dim3(blocks), dim3(LINES * SH::TPL), 0,
                          (hipStream_t)stream, *a);
       return (int)hipGetLastError();
     }
